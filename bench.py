@@ -1,0 +1,181 @@
+"""Benchmark of the bake-off operators on one MI355X.
+
+A step is one apply of the operator over the whole workload (inputs resident
+in HBM).  The headline workload is BK5 (collocated Laplacian) at p = 7:
+n = q = 8 GLL points per direction, E = 65536 elements of a 64 x 32 x 32
+perturbed box mesh (33.5 M DoF), six geometric factors per point, in FP64;
+u = uniform[-1, 1] + sin(pi x) sin(pi y) sin(pi z).  BK3 at p = 7 is timed
+beside it.  Rates are GDoF/s = E n^3 / t; algorithmic bytes are in + out +
+geometric factors (openmp/BK5.cpp:143-144, openmp/BK3.cpp:50-51).
+
+Prints one JSON line.  python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    return ap.parse_args()
+
+
+def time_steps(fn, steps, warmup):
+    """Average ms per step over `steps` calls of fn on the current stream,
+    bracketed by device synchronisation; HIP events on that stream."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    t0.record(s)
+    for _ in range(steps):
+        fn()
+    t1.record(s)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - wall0) * 1e3 / steps
+    return t0.elapsed_time(t1) / steps, wall
+
+
+def bk5_workload(p=7, E=65536):
+    import torch
+    from omp_bk_amd import geometry as geo
+    from omp_bk_amd import ops
+    dev = "cuda:0"
+    n = p + 1
+    torch.manual_seed(1234)
+    D = torch.as_tensor(geo.bk5_basis(p), device=dev)
+    G, xyz = geo.geometric_factors(5, p, E, device=dev, coords=True)
+    G = G.reshape(-1)
+    u = (torch.rand(E * n ** 3, dtype=torch.float64, device=dev) * 2 - 1) + geo.smooth_mode(xyz).reshape(-1)
+    del xyz
+    v = torch.empty_like(u)
+    fn = lambda: ops.bk5(p, D, G, u, out=v)
+    bytes_per_dof = 8 * (2 + 6)
+    return dict(fn=fn, dofs=E * n ** 3, bytes=bytes_per_dof * E * n ** 3, D=D, G=G, u=u, v=v, p=p, E=E)
+
+
+def bk3_workload(p=7, E=16384):
+    import torch
+    from omp_bk_amd import geometry as geo
+    from omp_bk_amd import ops
+    dev = "cuda:0"
+    n, q = p + 1, p + 2
+    torch.manual_seed(4321)
+    B, D = (torch.as_tensor(a, device=dev) for a in geo.bk13_bases(p))
+    G = geo.geometric_factors(3, p, E, device=dev).reshape(-1)
+    u = torch.rand(E * n ** 3, dtype=torch.float64, device=dev) * 2 - 1
+    v = torch.empty_like(u)
+    fn = lambda: ops.bk3(p, B, D, G, u, out=v)
+    return dict(fn=fn, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + 6 * E * q ** 3), p=p, E=E)
+
+
+def cpu_baseline(w, sample_elems=4096, budget_s=10.0):
+    """BK5 FP64 on the host: the reference's own SumFactorization<double, 8>
+    (oracle/_ref, compiled from the reference sources with g++ -O3 -fopenmp)
+    when present, else the oracle's restatement; min time over repeated
+    applies of a sample of the workload's elements."""
+    import numpy as np
+    p, E = w["p"], w["E"]
+    n = p + 1
+    ne = min(sample_elems, E)
+    D = np.ascontiguousarray(w["D"].cpu().numpy())
+    G = np.ascontiguousarray(w["G"][: ne * 6 * n ** 3].cpu().numpy())
+    u = np.ascontiguousarray(w["u"][: ne * n ** 3].cpu().numpy())
+    v = np.empty_like(u)
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libbkref.so")
+    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    if os.path.exists(ref_so):
+        lib, kind = ctypes.CDLL(ref_so), "reference"
+        call = lambda: lib.bkref_bk5_sumfact(ctypes.c_int(p), ctypes.c_size_t(ne), dp(D), dp(G), dp(u), dp(v))
+    elif os.path.exists(port_so):
+        lib, kind = ctypes.CDLL(port_so), "port"
+        call = lambda: lib.bko_bk5_sumfact(ctypes.c_int(n), ctypes.c_size_t(ne), dp(D), dp(G), dp(u), dp(v))
+    else:
+        return None
+    best, t_start, reps = float("inf"), time.perf_counter(), 0
+    while reps < 3 or (time.perf_counter() - t_start < budget_s and reps < 50):
+        t0 = time.perf_counter()
+        call()
+        best = min(best, time.perf_counter() - t0)
+        reps += 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(ne * n ** 3 / best * 1e-9, 4), "unit": "GDoF/s", "cores": threads, "kind": kind,
+            "sample": "BK5 p=%d FP64, first %d of the %d elements, min of %d applies" % (p, ne, E, reps)}
+
+
+def traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by a copy kernel), as recorded under profiles/."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        return json.load(open(path)).get(name, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    a = parse()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the operators are element-local and this build runs on a "
+              "single MI355X (--gpus 1).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    torch.cuda.init()
+
+    w5 = bk5_workload()
+    ms5, wall5 = time_steps(w5["fn"], a.steps, a.warmup)
+    gdofs5 = w5["dofs"] / (ms5 * 1e-3) * 1e-9
+    gbs5 = w5["bytes"] / (ms5 * 1e-3) * 1e-9
+
+    w3 = bk3_workload()
+    ms3, _ = time_steps(w3["fn"], a.steps, a.warmup)
+    gdofs3 = w3["dofs"] / (ms3 * 1e-3) * 1e-9
+    gbs3 = w3["bytes"] / (ms3 * 1e-3) * 1e-9
+    del w3
+
+    cpu = None if a.no_cpu else cpu_baseline(w5)
+
+    line = {
+        "metric": "BK3/BK5 GDoF/s at p=7, FP64 vs FP32-compensated at <=1e-12 rel err; % HBM roof",
+        "value": round(gdofs5, 3),
+        "unit": "GDoF/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms5, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: perturbed 64x32x32 box mesh, u = U[-1,1] + sin(pi x)sin(pi y)sin(pi z)",
+        "config": {"workload": "BK5 collocated Laplacian p=7 (n=q=8 GLL), E=65536, FP64 in/out/G",
+                   "p": 7, "nelmt": 65536, "dofs": w5["dofs"], "bytes_per_dof": 64},
+        "roofline": {"bound": "hbm", "achieved": round(gbs5, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs5 / HBM_PEAK_GBS, 4), "traffic": traffic("bk5_p7_f64")},
+        "cpu_baseline": cpu,
+        "wall_ms_per_step": round(wall5, 5),
+        "bk3_p7_f64": {"value": round(gdofs3, 3), "unit": "GDoF/s", "nelmt": 16384, "ms_per_step": round(ms3, 5),
+                       "achieved_GBs": round(gbs3, 1), "frac": round(gbs3 / HBM_PEAK_GBS, 4),
+                       "bytes_per_dof": round(8 * (2 + 6 * (9 / 8) ** 3), 3)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,87 @@
+/*
+ * bkx.h -- C ABI of the MI355X (gfx950) sum-factorised bake-off operators.
+ *
+ * Every entry point takes device pointers owned by the caller, a polynomial
+ * order p chosen at run time and a HIP stream (bkx_stream_t is a
+ * hipStream_t; NULL is the null stream), enqueues its kernels on that
+ * stream and returns a bkx_status without waiting for them.  The layouts are
+ * the reference's element-local E-vector layouts (row-major, last index
+ * fastest):
+ *
+ *   BK1/BK3  nm = p + 1 modes, nq = p + 2 Gauss points per direction;
+ *            in/out(e, i, j, k) of nm^3 per element;
+ *            basis B(i, p) = basis[i * nq + p] (nm x nq);
+ *            BK1 JxW(e, p, q, r) (nq^3 per element);
+ *            BK3 dbasis D(n, p) = dbasis[n * nq + p] (nq x nq),
+ *                G(e, f, p, q, r) factor-major, 6 factors rr rs rt ss st tt.
+ *   BK5      nq = p + 1 collocated points per direction;
+ *            in/out(e, i, j, k) of nq^3 per element;
+ *            dbasis D(i, n) = dbasis[i * nq + n] (row = output point);
+ *            G(e, i, j, f, k) interleaved, 6 factors rr rs rt ss st tt.
+ *
+ * The plain FP64 entry points are the drop-in for the reference's
+ * SumFactorization<double, nq> (the reference kernels return void and run
+ * synchronously; the C++ shim in bkx_sumfact.hpp restores that).
+ */
+#ifndef BKX_H
+#define BKX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef void* bkx_stream_t;
+
+enum bkx_status {
+    BKX_SUCCESS = 0,
+    BKX_ERROR_ORDER = 1,   /* p outside the operator's orders (the reference prints
+                              "unsupported polynomial order p", bk_common.h:163-178) */
+    BKX_ERROR_NULL = 2,    /* a required pointer is NULL while nelmt > 0 */
+    BKX_ERROR_LAUNCH = 3,  /* the kernel launch failed (hipGetLastError) */
+    BKX_ERROR_SIZE = 4,    /* nelmt beyond the largest grid */
+    BKX_ERROR_ARG = 5,     /* another invalid argument */
+    BKX_ERROR_DEVICE = 6   /* no usable HIP device or a HIP runtime call failed */
+};
+
+/* Human-readable name of a bkx_status. */
+const char* bkx_status_string(int status);
+
+/* 100 * major + minor of this library. */
+int bkx_version(void);
+
+/* Lowest and highest p of an operator: op = 1, 3 or 5. */
+int bkx_min_order(int op);
+int bkx_max_order(int op);
+
+/* ---------------------------------------------------------------------- */
+/* Plain FP64 operators (the drop-in)                                      */
+/* ---------------------------------------------------------------------- */
+
+/* BK1 mass operator, v = B^T diag(JxW) B u per element, p = 1..8.
+ * Replaces bk::SumFactorization<double, nq>(nelmt, basis, JxW, in, out)
+ * of openmp/bk1_sumfact.h:16-22 (kernel type sumfact_kernel<T>,
+ * openmp/bk1_sumfact.h:162-163). */
+int bkx_bk1_apply_f64(int p, size_t nelmt, const double* basis, const double* JxW, const double* in,
+                      double* out, bkx_stream_t stream);
+
+/* BK3 Poisson operator, v = B^T D^T G D B u per element, p = 1..8.
+ * Replaces bk::SumFactorization<double, nq>(nelmt, basis, dbasis, G, in,
+ * out) of openmp/bk3_sumfact.h:17-24 (kernel_fn of
+ * openmp/offload/bk3_offload.cpp:226). */
+int bkx_bk3_apply_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                      const double* in, double* out, bkx_stream_t stream);
+
+/* BK5 collocated Laplacian, v = D^T G D u per element, p = 1..7.
+ * Replaces bk::SumFactorization<double, nq>(nelmt, dbasis, G, in, out) of
+ * openmp/BK5.cpp:18-24 (kernel_fn of openmp/offload/bk5_offload.cpp:716). */
+int bkx_bk5_apply_f64(int p, size_t nelmt, const double* dbasis, const double* G, const double* in,
+                      double* out, bkx_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BKX_H */

@@ -1,0 +1,45 @@
+// bkx_device.hpp -- device-side helpers shared by the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace bkx {
+
+constexpr int kWave = 64;   // CDNA wavefront width
+
+// Orders LDS traffic between the lanes of one wavefront.  A wavefront's LDS
+// instructions are executed in issue order, so a lane may read what another
+// lane of the same wavefront wrote before this point without an s_barrier;
+// the wavefront-scope fences keep the compiler from moving LDS accesses
+// across it.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Keeps the computation of x at this point of the program: an empty asm
+// that reads and writes x in its VGPRs, so the arithmetic producing x cannot
+// be sunk past it (which would keep every layer's operands live at once).
+__device__ __forceinline__ void pin(double& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void pin(float& x) { asm volatile("" : "+v"(x)); }
+
+// Streaming loads and stores: every geometric factor and input value is used
+// by one element only, so they are marked non-temporal.
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T* p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+template <typename T>
+__device__ __forceinline__ void st_stream(T* p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
+}  // namespace bkx

@@ -1,0 +1,101 @@
+"""Host-side mirror of the reference's operator interface on torch tensors.
+
+The reference's kernels have the signatures
+  BK1  SumFactorization<T, nq>(nelmt, basis, JxW, in, out)       openmp/bk1_sumfact.h:16-22
+  BK3  SumFactorization<T, nq>(nelmt, basis, dbasis, G, in, out) openmp/bk3_sumfact.h:17-24
+  BK5  SumFactorization<T, nq>(nelmt, dbasis, G, in, out)        openmp/BK5.cpp:18-24
+with the order fixed at compile time.  Here the order p is an argument,
+the arrays are contiguous float64 tensors on one HIP device (flat or shaped,
+as long as their sizes match the element count), and the kernels run on the
+current torch stream through the C ABI of include/bkx.h.  Like the reference
+the calls return once the work is enqueued on the stream; torch's stream
+semantics order later work after it.
+"""
+import torch
+
+from . import _lib
+
+
+def _check(name, t, numel, dtype=torch.float64):
+    if not isinstance(t, torch.Tensor):
+        raise TypeError("%s must be a torch.Tensor" % name)
+    if t.device.type != "cuda":
+        raise ValueError("%s must be on a HIP device, got %s" % (name, t.device))
+    if t.dtype != dtype:
+        raise TypeError("%s must be %s, got %s" % (name, dtype, t.dtype))
+    if not t.is_contiguous():
+        raise ValueError("%s must be contiguous" % name)
+    if t.numel() != numel:
+        raise ValueError("%s has %d values, expected %d" % (name, t.numel(), numel))
+
+
+def _stream(t):
+    return ctypes_stream(torch.cuda.current_stream(t.device))
+
+
+def ctypes_stream(stream):
+    return stream.cuda_stream
+
+
+def _order_ok(op, p):
+    lo, hi = _lib.lib().bkx_min_order(op), _lib.lib().bkx_max_order(op)
+    if not (lo <= p <= hi):
+        raise ValueError("unsupported polynomial order p = %d (BK%d supports %d..%d)" % (p, op, lo, hi))
+
+
+def bk1(p, basis, JxW, u, out=None):
+    """BK1 mass operator v = B^T diag(JxW) B u per element (FP64)."""
+    _order_ok(1, p)
+    nm, nq = p + 1, p + 2
+    if u.numel() % nm ** 3:
+        raise ValueError("u must hold whole elements of %d values" % nm ** 3)
+    ne = u.numel() // nm ** 3
+    _check("basis", basis, nm * nq)
+    _check("JxW", JxW, ne * nq ** 3)
+    _check("u", u, ne * nm ** 3)
+    if out is None:
+        out = torch.empty_like(u)
+    _check("out", out, ne * nm ** 3)
+    with torch.cuda.device(u.device):
+        _lib.call("bkx_bk1_apply_f64", p, ne, basis.data_ptr(), JxW.data_ptr(), u.data_ptr(), out.data_ptr(),
+                  _stream(u))
+    return out
+
+
+def bk3(p, basis, dbasis, G, u, out=None):
+    """BK3 Poisson operator v = B^T D^T G D B u per element (FP64)."""
+    _order_ok(3, p)
+    nm, nq = p + 1, p + 2
+    if u.numel() % nm ** 3:
+        raise ValueError("u must hold whole elements of %d values" % nm ** 3)
+    ne = u.numel() // nm ** 3
+    _check("basis", basis, nm * nq)
+    _check("dbasis", dbasis, nq * nq)
+    _check("G", G, ne * 6 * nq ** 3)
+    _check("u", u, ne * nm ** 3)
+    if out is None:
+        out = torch.empty_like(u)
+    _check("out", out, ne * nm ** 3)
+    with torch.cuda.device(u.device):
+        _lib.call("bkx_bk3_apply_f64", p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), u.data_ptr(),
+                  out.data_ptr(), _stream(u))
+    return out
+
+
+def bk5(p, dbasis, G, u, out=None):
+    """BK5 collocated Laplacian v = D^T G D u per element (FP64)."""
+    _order_ok(5, p)
+    nq = p + 1
+    if u.numel() % nq ** 3:
+        raise ValueError("u must hold whole elements of %d values" % nq ** 3)
+    ne = u.numel() // nq ** 3
+    _check("dbasis", dbasis, nq * nq)
+    _check("G", G, ne * 6 * nq ** 3)
+    _check("u", u, ne * nq ** 3)
+    if out is None:
+        out = torch.empty_like(u)
+    _check("out", out, ne * nq ** 3)
+    with torch.cuda.device(u.device):
+        _lib.call("bkx_bk5_apply_f64", p, ne, dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(),
+                  _stream(u))
+    return out
