@@ -84,6 +84,29 @@ def bk3_workload(p=7, E=16384):
     return dict(fn=fn, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + 6 * E * q ** 3), p=p, E=E)
 
 
+def probe_roofs(nbytes=2 << 30, steps=20):
+    """Measured HBM roofs (GB/s, bytes read + written): a copy and the
+    read-heavy mixes R = 7 (BK5 FP64's 56 B read : 8 B written per DoF) and
+    R = 9 (the reference's read9)."""
+    import torch
+    from omp_bk_amd import _lib
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    c = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    st = lambda: torch.cuda.current_stream().cuda_stream
+    out = {}
+    ms, _ = time_steps(lambda: _lib.call("bkx_probe_copy", a.data_ptr(), c.data_ptr(), n, st()), steps, 3)
+    out["copy"] = round(2 * nbytes / (ms * 1e-3) * 1e-9, 1)
+    for r in (7, 9):
+        nout = (n // r) // 256 * 256
+        ms, _ = time_steps(lambda: _lib.call("bkx_probe_read_mix", r, a.data_ptr(), c.data_ptr(), nout, st()),
+                           steps, 3)
+        out["read%d" % r] = round(8 * nout * (r + 1) / (ms * 1e-3) * 1e-9, 1)
+    del a, c
+    torch.cuda.empty_cache()
+    return out
+
+
 def cpu_baseline(w, sample_elems=4096, budget_s=10.0):
     """BK5 FP64 on the host: the reference's own SumFactorization<double, 8>
     (oracle/_ref, compiled from the reference sources with g++ -O3 -fopenmp)
@@ -138,6 +161,7 @@ def main():
     import torch
     torch.cuda.init()
 
+    roofs = probe_roofs()
     w5 = bk5_workload()
     ms5, wall5 = time_steps(w5["fn"], a.steps, a.warmup)
     gdofs5 = w5["dofs"] / (ms5 * 1e-3) * 1e-9
@@ -169,6 +193,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs5, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs5 / HBM_PEAK_GBS, 4), "traffic": traffic("bk5_p7_f64")},
         "cpu_baseline": cpu,
+        "measured_roofs_GBs": roofs,
+        "frac_of_read7": round(gbs5 / roofs["read7"], 4),
         "wall_ms_per_step": round(wall5, 5),
         "bk3_p7_f64": {"value": round(gdofs3, 3), "unit": "GDoF/s", "nelmt": 16384, "ms_per_step": round(ms3, 5),
                        "achieved_GBs": round(gbs3, 1), "frac": round(gbs3 / HBM_PEAK_GBS, 4),

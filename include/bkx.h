@@ -80,6 +80,20 @@ int bkx_bk3_apply_f64(int p, size_t nelmt, const double* basis, const double* db
 int bkx_bk5_apply_f64(int p, size_t nelmt, const double* dbasis, const double* G, const double* in,
                       double* out, bkx_stream_t stream);
 
+/* ---------------------------------------------------------------------- */
+/* Bandwidth probes (the roofs the operators are measured against)         */
+/* ---------------------------------------------------------------------- */
+
+/* c = a for n doubles (n even).  HIP counterpart of the copy roof of
+ * roofline/stream_omp.cpp. */
+int bkx_probe_copy(const double* a, double* c, size_t n, bkx_stream_t stream);
+
+/* c[b*256 + l] = sum_{j<R} a[(b*R + j)*256 + l] for n_out doubles (a multiple
+ * of 256), R = reads_per_write in {1, 3, 7, 9}: reads R bytes per byte
+ * written.  R = 9 is the reference's read9 roof (roofline/stream_omp.cpp:
+ * 101-117); R = 7 matches BK5 in FP64 (56 B read, 8 B written per DoF). */
+int bkx_probe_read_mix(int reads_per_write, const double* a, double* c, size_t n_out, bkx_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif
