@@ -84,6 +84,34 @@ def bk3_workload(p=7, E=16384):
     return dict(fn=fn, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + 6 * E * q ** 3), p=p, E=E)
 
 
+def reduced_forms(w, steps, warmup, ms64):
+    """BK5 p=7 with u and G stored in FP32 (v in FP64): the float-float
+    compensated form f32c and the FP64-arithmetic form f32w.  Errors are
+    max|v - v64| / max|v64| against the FP64 operator on the same FP32-rounded
+    inputs (exact in FP64) and on the unrounded inputs."""
+    import torch
+    from omp_bk_amd import ops
+    p = w["p"]
+    G32, u32 = w["G"].float(), w["u"].float()
+    ref_q = ops.bk5(p, w["D"], G32.double(), u32.double())
+    out = {}
+    for form in ("f32c", "f32w"):
+        v = torch.empty_like(w["v"])
+        fn = lambda: ops.bk5(p, w["D"], G32, u32, out=v, form=form)
+        ms, _ = time_steps(fn, steps, warmup)
+        big = ref_q.abs().max()
+        out["bk5_p7_" + form] = {
+            "value": round(w["dofs"] / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
+            "speedup_vs_f64": round(ms64 / ms, 3), "bytes_per_dof": 36,
+            "frac": round(36 * w["dofs"] / (ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
+            "relerr_vs_f64_same_inputs": float(((v - ref_q).abs().max() / big).item()),
+            "relerr_vs_f64_unrounded_inputs": float(((v - w["v"]).abs().max() / w["v"].abs().max()).item()),
+        }
+        del v
+    del G32, u32, ref_q
+    return out
+
+
 def probe_roofs(nbytes=2 << 30, steps=20):
     """Measured HBM roofs (GB/s, bytes read + written): a copy and the
     read-heavy mixes R = 7 (BK5 FP64's 56 B read : 8 B written per DoF) and
@@ -167,6 +195,8 @@ def main():
     gdofs5 = w5["dofs"] / (ms5 * 1e-3) * 1e-9
     gbs5 = w5["bytes"] / (ms5 * 1e-3) * 1e-9
 
+    forms = reduced_forms(w5, a.steps, a.warmup, ms5)
+
     w3 = bk3_workload()
     ms3, _ = time_steps(w3["fn"], a.steps, a.warmup)
     gdofs3 = w3["dofs"] / (ms3 * 1e-3) * 1e-9
@@ -196,6 +226,7 @@ def main():
         "measured_roofs_GBs": roofs,
         "frac_of_read7": round(gbs5 / roofs["read7"], 4),
         "wall_ms_per_step": round(wall5, 5),
+        **forms,
         "bk3_p7_f64": {"value": round(gdofs3, 3), "unit": "GDoF/s", "nelmt": 16384, "ms_per_step": round(ms3, 5),
                        "achieved_GBs": round(gbs3, 1), "frac": round(gbs3 / HBM_PEAK_GBS, 4),
                        "bytes_per_dof": round(8 * (2 + 6 * (9 / 8) ** 3), 3)},

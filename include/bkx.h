@@ -46,6 +46,14 @@ enum bkx_status {
     BKX_ERROR_DEVICE = 6   /* no usable HIP device or a HIP runtime call failed */
 };
 
+/* Arithmetic forms of an operator (the tuning table is kept per form). */
+enum bkx_form {
+    BKX_FORM_F64 = 0,    /* FP64 storage and arithmetic (the drop-in) */
+    BKX_FORM_F32C = 1,   /* FP32 storage, float-float compensated arithmetic */
+    BKX_FORM_F32W = 2,   /* FP32 storage, FP64 arithmetic */
+    BKX_FORM_BF16S = 3   /* FP32 storage, split-BF16 MFMA products, FP32 accumulation */
+};
+
 /* Human-readable name of a bkx_status. */
 const char* bkx_status_string(int status);
 
@@ -55,6 +63,16 @@ int bkx_version(void);
 /* Lowest and highest p of an operator: op = 1, 3 or 5. */
 int bkx_min_order(int op);
 int bkx_max_order(int op);
+
+/* Launch variants (workgroup shape, elements per workgroup, load kind) of an
+ * operator form, and the one its entry points use at order p (0 until set).
+ * The autotuner (omp_bk_amd/tune.py) times every variant and sets the best;
+ * the reference picks variants with its study scripts instead
+ * (scripts/study/bk3_pick.sh).  Setting is process-wide; it is not
+ * synchronised with launches already enqueued. */
+int bkx_tune_count(int op, int form);
+int bkx_tune_set(int op, int form, int p, int variant);
+int bkx_tune_get(int op, int form, int p);
 
 /* ---------------------------------------------------------------------- */
 /* Plain FP64 operators (the drop-in)                                      */
@@ -79,6 +97,23 @@ int bkx_bk3_apply_f64(int p, size_t nelmt, const double* basis, const double* db
  * openmp/BK5.cpp:18-24 (kernel_fn of openmp/offload/bk5_offload.cpp:716). */
 int bkx_bk5_apply_f64(int p, size_t nelmt, const double* dbasis, const double* G, const double* in,
                       double* out, bkx_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Reduced-storage forms of BK5 (no reference counterpart: the reference    */
+/* runs BK5 in float or double only).  u and G are FP32, v is FP64.        */
+/* ---------------------------------------------------------------------- */
+
+/* FP32-compensated: float-float arithmetic throughout (D split into
+ * hi + lo, Dot2 compensated sums, the scheme of openmp/BK1_ff.cpp); v is
+ * joined to FP64.  Error vs the FP64 operator on the same (FP32) inputs is
+ * ~1e-14 relative. */
+int bkx_bk5_apply_f32c(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, double* out,
+                       bkx_stream_t stream);
+
+/* FP32 storage with FP64 arithmetic: the operator of bkx_bk5_apply_f64 on
+ * FP32 inputs (36 instead of 64 bytes per DoF). */
+int bkx_bk5_apply_f32w(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, double* out,
+                       bkx_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* Bandwidth probes (the roofs the operators are measured against)         */

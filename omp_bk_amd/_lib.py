@@ -18,8 +18,13 @@ _SIGNATURES = {
     "bkx_bk1_apply_f64": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk3_apply_f64": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk5_apply_f64": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk5_apply_f32c": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk5_apply_f32w": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_probe_copy": [_vp, _vp, _c_size, _vp],
     "bkx_probe_read_mix": [_c_int, _vp, _vp, _c_size, _vp],
+    "bkx_tune_count": [_c_int, _c_int],
+    "bkx_tune_set": [_c_int, _c_int, _c_int, _c_int],
+    "bkx_tune_get": [_c_int, _c_int, _c_int],
     "bkx_min_order": [_c_int],
     "bkx_max_order": [_c_int],
     "bkx_version": [],

@@ -74,7 +74,7 @@ __global__ __launch_bounds__((Bk13Shape<NM, NQ>::threads)) void bk13_f64_kernel(
         double ui[NM];
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            ui[i] = active ? ld_stream(u + ee * S::nm3 + i * S::nm2 + l) : 0.0;
+            ui[i] = active ? load<false>(u + ee * S::nm3 + i * S::nm2 + l) : 0.0;
         }
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {
@@ -111,6 +111,7 @@ __global__ __launch_bounds__((Bk13Shape<NM, NQ>::threads)) void bk13_f64_kernel(
 
     // ---- C: lane (q,r), contract k -> r; the quadrature-point operator ---
     double W[NQ];
+    double U[NQ];
     const int q = l / NQ;
     const int r = l - q * NQ;
     if (in_elem) {
@@ -119,7 +120,6 @@ __global__ __launch_bounds__((Bk13Shape<NM, NQ>::threads)) void bk13_f64_kernel(
         for (int k = 0; k < NM; ++k) {
             Br[k] = B[k * NQ + r];
         }
-        double U[NQ];
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {
             double t = 0.0;
@@ -134,7 +134,7 @@ __global__ __launch_bounds__((Bk13Shape<NM, NQ>::threads)) void bk13_f64_kernel(
             const double* je = Q + ee * S::nq3 + l;
 #pragma unroll
             for (int p = 0; p < NQ; ++p) {
-                W[p] = U[p] * (active ? ld_stream(je + p * S::nq2) : 0.0);
+                W[p] = U[p] * (active ? load<false>(je + p * S::nq2) : 0.0);
             }
         } else {
 #pragma unroll
@@ -162,7 +162,7 @@ __global__ __launch_bounds__((Bk13Shape<NM, NQ>::threads)) void bk13_f64_kernel(
                 double qr = 0.0, qs = 0.0, qt = 0.0;
 #pragma unroll
                 for (int n = 0; n < NQ; ++n) {
-                    qr += box0[X(n, q, r)] * D[n * NQ + p];
+                    qr += U[n] * D[n * NQ + p];
                 }
 #pragma unroll
                 for (int n = 0; n < NQ; ++n) {
@@ -175,7 +175,7 @@ __global__ __launch_bounds__((Bk13Shape<NM, NQ>::threads)) void bk13_f64_kernel(
                 double g[6];
 #pragma unroll
                 for (int f = 0; f < 6; ++f) {
-                    g[f] = active ? ld_stream(ge + f * S::nq3 + p * S::nq2) : 0.0;
+                    g[f] = active ? load<false>(ge + f * S::nq3 + p * S::nq2) : 0.0;
                 }
                 rr[p] = g[0] * qr + g[1] * qs + g[2] * qt;
                 box1[X(p, q, r)] = g[1] * qr + g[3] * qs + g[4] * qt;

@@ -1,0 +1,443 @@
+// bk5.hip -- BK5 collocated Laplacian with FP64 arithmetic on gfx950, with
+// u and G stored in FP64 (the drop-in) or in FP32 (half the bytes of the
+// geometric factors and of u; v is always written in FP64).
+//
+// Operator (openmp/BK5.cpp:18-103): per element, with D(i, n) = dbasis[i*nq+n]
+// and the six symmetric factors G(e, i, j, f, k),
+//   qr = sum_n D(i,n) u(n,j,k),  qs = sum_n D(j,n) u(i,n,k),  qt = sum_n D(k,n) u(i,j,n)
+//   (rr, rs, rt) = G(i,j,k) (qr, qs, qt)
+//   v(i,j,k) = sum_n D(n,i) rr(n,j,k) + D(n,j) rs(i,n,k) + D(n,k) rt(i,j,n).
+//
+// Mapping.  One wavefront per element (nq >= 6) or per group of
+// floor(64 / nq^2) elements (nq <= 5).  A lane owns the (j, k) column of its
+// element and keeps the whole i-line of u, of rr and of the result in
+// registers, so the i-direction contractions (qr and the rr transpose) are
+// register-only with D(i, n) uniform across the wavefront.  The element's u
+// is staged once in LDS for the j- and k-derivatives; rs and rt of one
+// i-layer at a time go through a small double-buffered LDS plane for the
+// transposed j- and k-contractions.  All LDS traffic stays inside one
+// wavefront, so there is no workgroup barrier; layers are processed in i
+// order so that only one layer of G (6 values per lane) is live at a time.
+//
+// HBM traffic per element: u once (s nq^3 B, s = sizeof storage), v once
+// (8 nq^3 B) and G once (6 s nq^3 B), all with unit-stride lanes: u/v planes
+// are nq^2 contiguous values, G(i, j, f, :) rows are nq contiguous values.
+#include "bkx_device.hpp"
+#include "bkx_launch.hpp"
+
+namespace bkx {
+namespace {
+
+template <int NQ>
+struct Bk5Shape {
+    static constexpr int nq2 = NQ * NQ;
+    static constexpr int nq3 = nq2 * NQ;
+    static constexpr int epw = kWave / nq2;       // elements per wavefront
+    static constexpr int row = NQ + 1;            // padded LDS row (bank spread)
+    static constexpr int plane = NQ * row;        // one padded (j, k) plane
+    static constexpr int box = NQ * plane;        // one padded element box
+    static_assert(epw >= 1, "one element must fit a wavefront");
+};
+
+template <int NQ, int WAVES, typename TS, bool NT>
+__global__ __launch_bounds__(kWave * WAVES) void bk5_kernel(size_t nelmt, const double* __restrict__ D,
+                                                           const TS* __restrict__ G, const TS* __restrict__ u,
+                                                           double* __restrict__ v)
+{
+    using S = Bk5Shape<NQ>;
+    constexpr int nq2 = S::nq2;
+    constexpr int nq3 = S::nq3;
+    constexpr int epw = S::epw;
+
+    __shared__ double s_u[WAVES][epw][S::box];
+    __shared__ double s_x[WAVES][2][2][epw][S::plane];
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int slot = lane / nq2;
+    const bool in_elem = slot < epw;           // idle lanes: nq^2 does not divide 64
+    const int sl = in_elem ? slot : epw - 1;   // idle lanes read (never write) a real slot
+    const int jk = lane - slot * nq2;
+    const int j = in_elem ? jk / NQ : 0;
+    const int k = in_elem ? jk - j * NQ : 0;
+    const size_t e = (size_t(blockIdx.x) * WAVES + wave) * epw + slot;
+    const bool active = in_elem && e < nelmt;
+    const size_t ee = active ? e : 0;
+    const int lx = j * S::row + k;             // this lane's slot in a padded plane
+
+    // Per-lane rows and columns of D for the j and k contractions.
+    double Dj[NQ], Dk[NQ], DTj[NQ], DTk[NQ];
+#pragma unroll
+    for (int n = 0; n < NQ; ++n) {
+        Dj[n] = D[j * NQ + n];
+        Dk[n] = D[k * NQ + n];
+        DTj[n] = D[n * NQ + j];
+        DTk[n] = D[n * NQ + k];
+    }
+
+    // u column along i -> registers and the padded LDS box.
+    const TS* ue = u + ee * nq3 + jk;
+    double* su = s_u[wave][sl];
+    double ur[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        ur[i] = active ? double(load<NT>(ue + i * nq2)) : 0.0;
+    }
+    if (in_elem) {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            su[i * S::plane + lx] = ur[i];
+        }
+    }
+    wave_sync();
+
+    // G(e, i, j, f, k): the lane's six factors of layer i are at
+    // ge + i * 6 nq^2 + f * nq.  One layer is prefetched ahead, kept in the
+    // storage type until it is used (a conversion at the load would wait for
+    // it there).
+    const TS* ge = G + ee * 6 * nq3 + j * 6 * NQ + k;
+    TS gn[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        gn[f] = active ? load<NT>(ge + f * NQ) : TS(0);
+    }
+    // acc(m) = v(m, j, k): the i-direction transpose sum_i D(i, m) rr(i) is
+    // accumulated as each layer's rr appears (i ascending), the j/k
+    // transposes of layer m are added at layer m.
+    double acc[NQ];
+#pragma unroll
+    for (int m = 0; m < NQ; ++m) {
+        acc[m] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        double g[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            g[f] = double(gn[f]);
+        }
+        if (i + 1 < NQ) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                gn[f] = active ? load<NT>(ge + (i + 1) * 6 * nq2 + f * NQ) : TS(0);
+            }
+        }
+        double qr = 0.0, qs = 0.0, qt = 0.0;
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+            qr += D[i * NQ + n] * ur[n];
+        }
+        const double* sp = su + i * S::plane;
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+            qs += Dj[n] * sp[n * S::row + k];
+        }
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+            qt += Dk[n] * sp[j * S::row + n];
+        }
+        const double rr = g[0] * qr + g[1] * qs + g[2] * qt;
+        const double rs = g[1] * qr + g[3] * qs + g[4] * qt;
+        const double rt = g[2] * qr + g[4] * qs + g[5] * qt;
+
+        double* xs = s_x[wave][i & 1][0][sl];
+        double* xt = s_x[wave][i & 1][1][sl];
+        if (in_elem) {
+            xs[lx] = rs;
+            xt[lx] = rt;
+        }
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+            acc[m] += D[i * NQ + m] * rr;
+        }
+        wave_sync();
+        double a = acc[i];
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+            a += DTj[n] * xs[n * S::row + k];
+        }
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+            a += DTk[n] * xt[j * S::row + n];
+        }
+        acc[i] = a;
+#pragma unroll
+        for (int m = 0; m < NQ; ++m) {
+            pin(acc[m]);
+        }
+    }
+
+    if (active) {
+        double* ve = v + e * nq3 + jk;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            st_stream(ve + i * nq2, acc[i]);
+        }
+    }
+}
+
+// Transposing mapping.  The same wavefront per element, but every
+// contraction runs along the lane's register column with D(., .) uniform
+// across the wavefront (scalar operands); the LDS is used only to move the
+// element between three layouts, lane (j,k) / (i,k) / (i,j) holding the
+// column along i / j / k:
+//   u (L_i) -> LDS;  qr in L_i;  u read as L_j -> qs;  as L_k -> qt
+//   qs, qt -> LDS -> read in L_i;  fluxes with G (L_i), rr kept
+//   rs, rt -> LDS -> read in L_j / L_k -> transposed j / k contractions
+//   -> LDS -> read in L_i and added to the transposed i contraction of rr.
+// Per element that is 7 nq^3 LDS writes and 8 nq^3 reads, against
+// 3 nq^3 writes and 4 nq^4 reads when the j and k contractions read their
+// operands from LDS (the slab mapping above).  MINW is the occupancy
+// (wavefronts per SIMD) the register allocation is held to.
+template <int NQ, int WAVES, typename TS, bool NT, int MINW>
+__global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(MINW))) void bk5t_kernel(size_t nelmt, const double* __restrict__ D,
+                                                            const TS* __restrict__ G, const TS* __restrict__ u,
+                                                            double* __restrict__ v)
+{
+    using S = Bk5Shape<NQ>;
+    constexpr int nq2 = S::nq2;
+    constexpr int nq3 = S::nq3;
+    constexpr int epw = S::epw;
+
+    __shared__ double s_a[WAVES][epw][S::box];
+    __shared__ double s_b[WAVES][epw][S::box];
+
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int slot = lane / nq2;
+    const bool in_elem = slot < epw;
+    const int sl = in_elem ? slot : epw - 1;
+    const int ab = lane - slot * nq2;
+    const int a = in_elem ? ab / NQ : 0;
+    const int b = in_elem ? ab - a * NQ : 0;
+    const size_t e = (size_t(blockIdx.x) * WAVES + wave) * epw + slot;
+    const bool active = in_elem && e < nelmt;
+    const size_t ee = active ? e : 0;
+    double* sa = s_a[wave][sl];
+    double* sb = s_b[wave][sl];
+    // padded box address of (i, j, k)
+    auto X = [](int i, int j, int k) { return i * S::plane + j * S::row + k; };
+
+    // u in L_i: lane (j, k) = (a, b), column i
+    const TS* ue = u + ee * nq3 + a * NQ + b;
+    double x[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        x[c] = active ? double(load<NT>(ue + c * nq2)) : 0.0;
+    }
+    // first layer of G
+    const TS* ge = G + ee * 6 * nq3 + a * 6 * NQ + b;
+    TS gn[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        gn[f] = active ? load<NT>(ge + f * NQ) : TS(0);
+    }
+    if (in_elem) {
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            sa[X(c, a, b)] = x[c];
+        }
+    }
+    wave_sync();
+    // one pass over the rows of D: qr (L_i, u in registers), qs in L_j
+    // (lane (i, k) = (a, b)) and qt in L_k (lane (i, j) = (a, b))
+    double qr[NQ], qs[NQ], qt[NQ];
+    {
+        const auto Ds = launder<2>(D);
+        double y[NQ], z[NQ];
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+            y[n] = sa[X(a, n, b)];
+            z[n] = sa[X(a, b, n)];
+        }
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            double tr = 0.0, ts = 0.0, tt = 0.0;
+#pragma unroll
+            for (int n = 0; n < NQ; ++n) {
+                tr += Ds[c * NQ + n] * x[n];
+                ts += Ds[c * NQ + n] * y[n];
+                tt += Ds[c * NQ + n] * z[n];
+            }
+            qr[c] = tr;
+            qs[c] = ts;
+            qt[c] = tt;
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    wave_sync();
+    if (in_elem) {
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            sb[X(a, c, b)] = qs[c];
+            sa[X(a, b, c)] = qt[c];
+        }
+    }
+    wave_sync();
+    // fluxes in L_i, one layer of G at a time (prefetched one ahead); rs and
+    // rt overwrite the qs and qt this lane has just read
+    double rr[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        double g[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            g[f] = double(gn[f]);
+        }
+        if (c + 1 < NQ) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                gn[f] = active ? load<NT>(ge + (c + 1) * 6 * nq2 + f * NQ) : TS(0);
+            }
+        }
+        const double s_ = sb[X(c, a, b)];
+        const double t_ = sa[X(c, a, b)];
+        const double r_ = qr[c];
+        rr[c] = g[0] * r_ + g[1] * s_ + g[2] * t_;
+        const double rs = g[1] * r_ + g[3] * s_ + g[4] * t_;
+        const double rt = g[2] * r_ + g[4] * s_ + g[5] * t_;
+        if (in_elem) {
+            sb[X(c, a, b)] = rs;
+            sa[X(c, a, b)] = rt;
+        }
+    }
+    wave_sync();
+    // transposed j contraction in L_j (lane (i, k)), k contraction in L_k (lane (i, j))
+    // (row n of D times y[n], accumulated over n ascending: one contiguous row
+    // of scalar operands per step)
+    double wj[NQ], wk[NQ];
+    {
+        const auto Dt = launder<3>(D);
+        double y[NQ], z[NQ];
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+            y[n] = sb[X(a, n, b)];
+            z[n] = sa[X(a, b, n)];
+        }
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            wj[c] = 0.0;
+            wk[c] = 0.0;
+        }
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) {
+                wj[c] += Dt[n * NQ + c] * y[n];
+                wk[c] += Dt[n * NQ + c] * z[n];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+    wave_sync();
+    if (in_elem) {
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            sb[X(a, c, b)] = wj[c];
+            sa[X(a, b, c)] = wk[c];
+        }
+    }
+    wave_sync();
+    if (active) {
+        const auto Df = launder<4>(D);
+        double t[NQ];
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            t[c] = 0.0;
+        }
+#pragma unroll
+        for (int n = 0; n < NQ; ++n) {
+#pragma unroll
+            for (int c = 0; c < NQ; ++c) {
+                t[c] += Df[n * NQ + c] * rr[n];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        double* ve = v + e * nq3 + a * NQ + b;
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            st_stream(ve + c * nq2, t[c] + sb[X(c, a, b)] + sa[X(c, a, b)]);
+        }
+    }
+}
+
+template <int NQ, int WAVES, typename TS, bool NT, int MINW = 1>
+int launch_bk5t_v(size_t nelmt, const double* D, const TS* G, const TS* u, double* v, hipStream_t s)
+{
+    constexpr size_t per_block = size_t(WAVES) * Bk5Shape<NQ>::epw;
+    const unsigned grid = grid_for(nelmt, per_block);
+    if (grid == 0) {
+        return BKX_ERROR_SIZE;
+    }
+    hipLaunchKernelGGL((bk5t_kernel<NQ, WAVES, TS, NT, MINW>), dim3(grid), dim3(kWave * WAVES), 0, s, nelmt, D, G, u, v);
+    return launch_status();
+}
+
+template <int NQ, int WAVES, typename TS, bool NT>
+int launch_bk5_v(size_t nelmt, const double* D, const TS* G, const TS* u, double* v, hipStream_t s)
+{
+    constexpr size_t per_block = size_t(WAVES) * Bk5Shape<NQ>::epw;
+    const unsigned grid = grid_for(nelmt, per_block);
+    if (grid == 0) {
+        return BKX_ERROR_SIZE;
+    }
+    hipLaunchKernelGGL((bk5_kernel<NQ, WAVES, TS, NT>), dim3(grid), dim3(kWave * WAVES), 0, s, nelmt, D, G, u, v);
+    return launch_status();
+}
+
+// Launch variants (the autotuner's search space): the slab mapping (0-3) or
+// the transposing mapping (4-8), wavefronts per workgroup, ordinary or
+// non-temporal loads, and an occupancy target.  Ordinary loads come first:
+// on MI355X the non-temporal hint costs 10% (p = 7) to 40% (p = 3), as a
+// G row of one factor is 64 contiguous bytes and the next factor's row
+// fills the rest of the same 128-byte line.
+constexpr int kBk5Variants = 9;
+
+template <int NQ, typename TS>
+int launch_bk5(int variant, size_t nelmt, const double* D, const TS* G, const TS* u, double* v, hipStream_t s)
+{
+    switch (variant) {
+    case 0:
+        return launch_bk5_v<NQ, 4, TS, false>(nelmt, D, G, u, v, s);
+    case 1:
+        return launch_bk5_v<NQ, 1, TS, false>(nelmt, D, G, u, v, s);
+    case 2:
+        return launch_bk5_v<NQ, 2, TS, false>(nelmt, D, G, u, v, s);
+    case 3:
+        return launch_bk5_v<NQ, 4, TS, true>(nelmt, D, G, u, v, s);
+    case 4:
+        return launch_bk5t_v<NQ, 4, TS, false>(nelmt, D, G, u, v, s);
+    case 5:
+        return launch_bk5t_v<NQ, 1, TS, false>(nelmt, D, G, u, v, s);
+    case 6:
+        return launch_bk5t_v<NQ, 2, TS, false>(nelmt, D, G, u, v, s);
+    case 7:
+        return launch_bk5t_v<NQ, 1, TS, true>(nelmt, D, G, u, v, s);
+    case 8:
+        return launch_bk5t_v<NQ, 4, TS, false, 3>(nelmt, D, G, u, v, s);
+    default:
+        return BKX_ERROR_ARG;
+    }
+}
+
+}  // namespace
+
+int bk5_variants() { return kBk5Variants; }
+
+int bk5_apply_f64(int variant, int p, size_t nelmt, const double* D, const double* G, const double* u, double* v,
+                  hipStream_t s)
+{
+    return with_order<1, 7>(p, [&](auto P) {
+        return launch_bk5<decltype(P)::value + 1>(variant, nelmt, D, G, u, v, s);
+    });
+}
+
+int bk5_apply_f32w(int variant, int p, size_t nelmt, const double* D, const float* G, const float* u, double* v,
+                   hipStream_t s)
+{
+    return with_order<1, 7>(p, [&](auto P) {
+        return launch_bk5<decltype(P)::value + 1>(variant, nelmt, D, G, u, v, s);
+    });
+}
+
+}  // namespace bkx

@@ -2,6 +2,8 @@
 // launch of the operator's kernel on the caller's stream.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "bkx.h"
 #include "bkx_launch.hpp"
 
@@ -10,8 +12,13 @@ int bk1_apply_f64(int p, size_t nelmt, const double* B, const double* JxW, const
                   hipStream_t s);
 int bk3_apply_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* u,
                   double* v, hipStream_t s);
-int bk5_apply_f64(int p, size_t nelmt, const double* D, const double* G, const double* u, double* v,
+int bk5_variants();
+int bk5_apply_f64(int variant, int p, size_t nelmt, const double* D, const double* G, const double* u, double* v,
                   hipStream_t s);
+int bk5_apply_f32c(int p, size_t nelmt, const double* D, const float* G, const float* u, double* v,
+                   hipStream_t s);
+int bk5_apply_f32w(int variant, int p, size_t nelmt, const double* D, const float* G, const float* u, double* v,
+                   hipStream_t s);
 }  // namespace bkx
 
 namespace {
@@ -19,6 +26,20 @@ namespace {
 bool order_ok(int op, int p) { return p >= bkx_min_order(op) && p <= bkx_max_order(op); }
 
 hipStream_t as_stream(bkx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Launch variant of each (operator, form, order), set by the autotuner;
+// variant 0 until then.
+constexpr int kOps = 6;
+constexpr int kForms = 4;
+constexpr int kOrders = 9;
+std::atomic<int> g_variant[kOps][kForms][kOrders];
+
+bool tune_index_ok(int op, int form, int p)
+{
+    return (op == 1 || op == 3 || op == 5) && form >= 0 && form < kForms && p >= 0 && p < kOrders;
+}
+
+int variant_of(int op, int form, int p) { return g_variant[op][form][p].load(std::memory_order_relaxed); }
 
 }  // namespace
 
@@ -47,6 +68,31 @@ const char* bkx_status_string(int status)
 }
 
 int bkx_version(void) { return 100; }
+
+int bkx_tune_count(int op, int form)
+{
+    if (op == 5 && (form == BKX_FORM_F64 || form == BKX_FORM_F32W)) {
+        return bkx::bk5_variants();
+    }
+    return (op == 1 || op == 3 || op == 5) && form >= 0 && form < kForms ? 1 : -1;
+}
+
+int bkx_tune_set(int op, int form, int p, int variant)
+{
+    if (!tune_index_ok(op, form, p)) {
+        return BKX_ERROR_ARG;
+    }
+    if (variant < 0 || variant >= bkx_tune_count(op, form)) {
+        return BKX_ERROR_ARG;
+    }
+    g_variant[op][form][p].store(variant, std::memory_order_relaxed);
+    return BKX_SUCCESS;
+}
+
+int bkx_tune_get(int op, int form, int p)
+{
+    return tune_index_ok(op, form, p) ? variant_of(op, form, p) : -1;
+}
 
 int bkx_min_order(int op) { return (op == 1 || op == 3 || op == 5) ? 1 : -1; }
 
@@ -100,7 +146,37 @@ int bkx_bk5_apply_f64(int p, size_t nelmt, const double* dbasis, const double* G
     if (!dbasis || !G || !in || !out) {
         return BKX_ERROR_NULL;
     }
-    return bkx::bk5_apply_f64(p, nelmt, dbasis, G, in, out, as_stream(stream));
+    return bkx::bk5_apply_f64(variant_of(5, BKX_FORM_F64, p), p, nelmt, dbasis, G, in, out, as_stream(stream));
+}
+
+int bkx_bk5_apply_f32c(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, double* out,
+                       bkx_stream_t stream)
+{
+    if (!order_ok(5, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!dbasis || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk5_apply_f32c(p, nelmt, dbasis, G, in, out, as_stream(stream));
+}
+
+int bkx_bk5_apply_f32w(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, double* out,
+                       bkx_stream_t stream)
+{
+    if (!order_ok(5, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!dbasis || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk5_apply_f32w(variant_of(5, BKX_FORM_F32W, p), p, nelmt, dbasis, G, in, out, as_stream(stream));
 }
 
 }  // extern "C"

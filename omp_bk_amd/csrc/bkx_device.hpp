@@ -32,12 +32,47 @@ __device__ __forceinline__ void wave_sync()
 __device__ __forceinline__ void pin(double& x) { asm volatile("" : "+v"(x)); }
 __device__ __forceinline__ void pin(float& x) { asm volatile("" : "+v"(x)); }
 
+// The same uniform pointer, opaque to the optimiser: loads through it in
+// different program phases are not merged, so a small wavefront-uniform
+// matrix is re-read from the scalar cache per phase instead of being kept
+// live (in scalar, then vector registers) across the whole kernel.
+// The asm is not volatile and has no memory clobber, so it does not count as
+// a store that could clobber the loads after it (which must stay scalar
+// loads); TAG makes each phase's asm text distinct so that two launders of
+// the same pointer are not merged.  An inline-asm result counts as
+// divergent, so the pointer is made provably uniform again with
+// readfirstlane, and it is returned in the global address space (a generic
+// pointer would be read with vector flat loads).
+template <typename T>
+using global_ptr = const T __attribute__((address_space(1)))*;
+
+template <int TAG, typename T>
+__device__ __forceinline__ global_ptr<T> launder(const T* p)
+{
+    asm("; launder %1" : "+s"(p) : "i"(TAG));
+    const uint64_t x = reinterpret_cast<uint64_t>(p);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
+    const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));
+    return (global_ptr<T>)(reinterpret_cast<const T*>((uint64_t(hi) << 32) | lo));
+}
+
 // Streaming loads and stores: every geometric factor and input value is used
 // by one element only, so they are marked non-temporal.
 template <typename T>
 __device__ __forceinline__ T ld_stream(const T* p)
 {
     return __builtin_nontemporal_load(p);
+}
+
+// A load that is non-temporal when NT is set.
+template <bool NT, typename T>
+__device__ __forceinline__ T load(const T* p)
+{
+    if constexpr (NT) {
+        return __builtin_nontemporal_load(p);
+    } else {
+        return *p;
+    }
 }
 
 template <typename T>

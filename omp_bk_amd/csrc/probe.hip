@@ -12,13 +12,27 @@ namespace {
 
 constexpr int kProbeThreads = 256;
 
-// c = a, 16 B per lane per access, grid-stride
+// c = a: each workgroup copies kProbeUnroll consecutive 16-byte vectors per
+// lane (lanes adjacent within each), one workgroup per 256 * kProbeUnroll
+// vectors; the tail is guarded.
+constexpr int kProbeUnroll = 4;
+
 __global__ __launch_bounds__(kProbeThreads) void copy_kernel(const f64x2* __restrict__ a, f64x2* __restrict__ c,
                                                              size_t n2)
 {
-    const size_t stride = size_t(gridDim.x) * blockDim.x;
-    for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n2; i += stride) {
-        st_stream(c + i, ld_stream(a + i));
+    const size_t base = size_t(blockIdx.x) * kProbeThreads * kProbeUnroll + threadIdx.x;
+    f64x2 x[kProbeUnroll];
+#pragma unroll
+    for (int t = 0; t < kProbeUnroll; ++t) {
+        const size_t i = base + size_t(t) * kProbeThreads;
+        x[t] = i < n2 ? ld_stream(a + i) : f64x2{0.0, 0.0};
+    }
+#pragma unroll
+    for (int t = 0; t < kProbeUnroll; ++t) {
+        const size_t i = base + size_t(t) * kProbeThreads;
+        if (i < n2) {
+            st_stream(c + i, x[t]);
+        }
     }
 }
 
@@ -29,29 +43,22 @@ __global__ __launch_bounds__(kProbeThreads) void read_mix_kernel(const f64x2* __
                                                                  f64x2* __restrict__ c, size_t nblk)
 {
     constexpr int half = 128;   // f64x2 per 256-double block
-    const size_t stride = size_t(gridDim.x) * (blockDim.x / half);
     const int l = threadIdx.x % half;
-    for (size_t blk = size_t(blockIdx.x) * (blockDim.x / half) + threadIdx.x / half; blk < nblk; blk += stride) {
+    const size_t blk = size_t(blockIdx.x) * (kProbeThreads / half) + threadIdx.x / half;
+    if (blk < nblk) {
+        f64x2 x[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            x[j] = ld_stream(a + (blk * R + j) * half + l);
+        }
         f64x2 s = f64x2{0.0, 0.0};
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            const f64x2 x = ld_stream(a + (blk * R + j) * half + l);
-            s.x += x.x;
-            s.y += x.y;
+            s.x += x[j].x;
+            s.y += x[j].y;
         }
         st_stream(c + blk * half + l, s);
     }
-}
-
-unsigned probe_grid(size_t work_items)
-{
-    int dev = 0;
-    int cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) {
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    }
-    const size_t want = (work_items + kProbeThreads - 1) / kProbeThreads;
-    return unsigned(std::min<size_t>(want, size_t(cus) * 16));
 }
 
 }  // namespace
@@ -70,7 +77,11 @@ int bkx_probe_copy(const double* a, double* c, size_t n, bkx_stream_t stream)
     if (n % 2) {
         return BKX_ERROR_ARG;
     }
-    const unsigned grid = bkx::probe_grid(n / 2);
+    const size_t per_block = size_t(bkx::kProbeThreads) * bkx::kProbeUnroll;
+    const unsigned grid = bkx::grid_for(n / 2, per_block);
+    if (grid == 0) {
+        return BKX_ERROR_SIZE;
+    }
     hipLaunchKernelGGL(bkx::copy_kernel, dim3(grid), dim3(bkx::kProbeThreads), 0,
                        reinterpret_cast<hipStream_t>(stream), reinterpret_cast<const bkx::f64x2*>(a),
                        reinterpret_cast<bkx::f64x2*>(c), n / 2);
@@ -89,7 +100,10 @@ int bkx_probe_read_mix(int reads_per_write, const double* a, double* c, size_t n
         return BKX_ERROR_ARG;
     }
     const size_t nblk = n_out / 256;
-    const unsigned grid = bkx::probe_grid(n_out / 2);
+    const unsigned grid = bkx::grid_for(nblk, bkx::kProbeThreads / 128);
+    if (grid == 0) {
+        return BKX_ERROR_SIZE;
+    }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     const bkx::f64x2* a2 = reinterpret_cast<const bkx::f64x2*>(a);
     bkx::f64x2* c2 = reinterpret_cast<bkx::f64x2*>(c);

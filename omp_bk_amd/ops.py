@@ -82,20 +82,34 @@ def bk3(p, basis, dbasis, G, u, out=None):
     return out
 
 
-def bk5(p, dbasis, G, u, out=None):
-    """BK5 collocated Laplacian v = D^T G D u per element (FP64)."""
+_BK5_FORMS = {
+    # form: (C entry point, storage dtype of u and G)
+    "f64": ("bkx_bk5_apply_f64", torch.float64),
+    "f32c": ("bkx_bk5_apply_f32c", torch.float32),
+    "f32w": ("bkx_bk5_apply_f32w", torch.float32),
+}
+
+
+def bk5(p, dbasis, G, u, out=None, form="f64"):
+    """BK5 collocated Laplacian v = D^T G D u per element.
+
+    form "f64": u, G, v in FP64 (the drop-in);
+         "f32c": u, G in FP32, float-float compensated arithmetic, v FP64;
+         "f32w": u, G in FP32, FP64 arithmetic, v FP64."""
     _order_ok(5, p)
+    if form not in _BK5_FORMS:
+        raise ValueError("unknown BK5 form %r (one of %s)" % (form, ", ".join(_BK5_FORMS)))
+    entry, sdt = _BK5_FORMS[form]
     nq = p + 1
     if u.numel() % nq ** 3:
         raise ValueError("u must hold whole elements of %d values" % nq ** 3)
     ne = u.numel() // nq ** 3
     _check("dbasis", dbasis, nq * nq)
-    _check("G", G, ne * 6 * nq ** 3)
-    _check("u", u, ne * nq ** 3)
+    _check("G", G, ne * 6 * nq ** 3, sdt)
+    _check("u", u, ne * nq ** 3, sdt)
     if out is None:
-        out = torch.empty_like(u)
+        out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nq ** 3)
     with torch.cuda.device(u.device):
-        _lib.call("bkx_bk5_apply_f64", p, ne, dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(),
-                  _stream(u))
+        _lib.call(entry, p, ne, dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
     return out

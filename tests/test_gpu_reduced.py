@@ -1,0 +1,89 @@
+"""Reduced-storage BK5 forms against the FP64 oracle on the same quantised
+inputs (u and G rounded to FP32 are exact FP64 values, so the oracle is the
+exact operator of the stored data up to FP64 rounding).
+
+Bars: f32c (float-float arithmetic) <= 1e-12 max-relative, the north-star
+bound for the FP32-compensated form; f32w (FP64 arithmetic) <= 1e-13.
+"""
+import numpy as np
+import pytest
+import torch
+
+import _oracle as O
+from omp_bk_amd import geometry as geo
+from omp_bk_amd import ops
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+BARS = {"f32c": 1e-12, "f32w": 1e-13}
+
+
+def _case(p, E, seed):
+    n = p + 1
+    rng = np.random.default_rng(seed)
+    shape = geo.box_shape(E) if (E & (E - 1)) == 0 else (E, 1, 1)
+    G, xyz = geo.geometric_factors(5, p, E, shape=shape, coords=True)
+    u = rng.uniform(-1, 1, E * n ** 3) + geo.smooth_mode(xyz).reshape(-1)
+    return geo.bk5_basis(p), G.reshape(-1), u
+
+
+def _run(form, p, D, G, u):
+    n = u.size
+    out = torch.full((n,), float("nan"), dtype=torch.float64, device=DEV)
+    ops.bk5(p, torch.as_tensor(D, device=DEV), torch.as_tensor(G.astype(np.float32), device=DEV),
+            torch.as_tensor(u.astype(np.float32), device=DEV), out=out, form=form)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    assert np.all(np.isfinite(got))
+    return got
+
+
+@pytest.mark.parametrize("form", ["f32c", "f32w"])
+@pytest.mark.parametrize("p", range(1, 8))
+@pytest.mark.parametrize("E", [1, 5, 64])
+def test_against_quantised_oracle(form, p, E):
+    D, G, u = _case(p, E, seed=p * 31 + E)
+    Gq = G.astype(np.float32).astype(np.float64)
+    uq = u.astype(np.float32).astype(np.float64)
+    ref = O.bk5(p + 1, D, Gq, uq)
+    got = _run(form, p, D, G, u)
+    assert O.rel_maxdiff(got, ref) <= BARS[form]
+
+
+@pytest.mark.parametrize("form", ["f32c", "f32w"])
+def test_lcg_data(form):
+    """The reference's LCG data: values that are exact in FP32 already."""
+    p, E = 7, 9
+    n = p + 1
+    u, G = O.fill_random(E * n ** 3, E * 6 * n ** 3, seed=77)
+    G = G.astype(np.float32).astype(np.float64)
+    D = O.test_basis(n, n)
+    got = _run(form, p, D, G, u)
+    assert O.rel_maxdiff(got, O.bk5(n, D, G, u)) <= BARS[form]
+
+
+@pytest.mark.parametrize("form", ["f32c", "f32w"])
+def test_full_size_sampled(form):
+    """p = 7, E = 65536: sampled elements against the quantised oracle, and
+    the error against the unquantised FP64 operator (storage rounding only)."""
+    p, E = 7, 65536
+    n = p + 1
+    D = torch.as_tensor(geo.bk5_basis(p), device=DEV)
+    G, xyz = geo.geometric_factors(5, p, E, device=DEV, coords=True)
+    G = G.reshape(-1)
+    torch.manual_seed(3)
+    u = (torch.rand(E * n ** 3, dtype=torch.float64, device=DEV) * 2 - 1) + geo.smooth_mode(xyz).reshape(-1)
+    del xyz
+    G32, u32 = G.float(), u.float()
+    v = ops.bk5(p, D, G32, u32, form=form)
+    v64 = ops.bk5(p, D, G, u)
+    idx = torch.as_tensor(np.random.default_rng(2).choice(E, 256, replace=False), device=DEV)
+    Gs = G32.view(E, -1)[idx].double().cpu().numpy()
+    us = u32.view(E, -1)[idx].double().cpu().numpy()
+    ref = O.bk5(n, D.cpu().numpy(), Gs, us)
+    got = v.view(E, -1)[idx].cpu().numpy().reshape(-1)
+    assert O.rel_maxdiff(got, ref) <= BARS[form]
+    # against unquantised inputs the error is the FP32 rounding of u and G
+    e64 = ((v - v64).abs().max() / v64.abs().max()).item()
+    assert 1e-10 < e64 < 1e-5
