@@ -116,6 +116,34 @@ int bkx_bk5_apply_f32w(int p, size_t nelmt, const double* dbasis, const float* G
                        bkx_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
+/* BK3 with a coefficient and its adjoint (no reference counterpart; the    */
+/* operator is that of bkx_bk3_apply_f64 with each point's six factors      */
+/* scaled by kappa(e, p, q, r), laid out like BK1's JxW).                  */
+/* ---------------------------------------------------------------------- */
+
+/* out = A(kappa) in. */
+int bkx_bk3_apply_kappa_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                            const double* kappa, const double* in, double* out, bkx_stream_t stream);
+
+/* For L = 1/2 |A(kappa) u - b|^2: dldu = A(kappa) r and
+ * dldkappa(e,p,q,r) = (grad r) . G (grad u) at each Gauss point, with
+ * r = A(kappa) u - b and grad the reference gradient (qr, qs, qt); loss[e] =
+ * 1/2 |r_e|^2 per element (sum them with bkx_sum_f64 for L).  One fused
+ * kernel; no quadrature-point data is stored. */
+int bkx_bk3_adjoint_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                        const double* kappa, const double* u, const double* b, double* dldu, double* dldkappa,
+                        double* loss, bkx_stream_t stream);
+
+/* Vector-Jacobian product of v = A(kappa) u for a cotangent w:
+ * grad_u = A(kappa) w, grad_kappa = (grad w) . G (grad u). */
+int bkx_bk3_vjp_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                    const double* kappa, const double* u, const double* w, double* grad_u, double* grad_kappa,
+                    bkx_stream_t stream);
+
+/* *out = sum of x[0..n-1] in a fixed order (one workgroup; deterministic). */
+int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
 /* Bandwidth probes (the roofs the operators are measured against)         */
 /* ---------------------------------------------------------------------- */
 

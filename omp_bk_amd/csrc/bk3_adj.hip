@@ -1,0 +1,237 @@
+// bk3_adj.hip -- BK3 with a per-quadrature-point coefficient kappa scaling
+// the six geometric factors, and its adjoint, in FP64 on gfx950.
+//
+//   A(kappa) u = sum_x P_x^T kappa_x G_x P_x u   per element, where P_x u is
+//   the reference gradient (qr, qs, qt) at Gauss point x (the operator of
+//   openmp/bk3_sumfact.h:17-187 with G_x replaced by kappa_x G_x).
+//
+// A(kappa) is symmetric, so for L = 1/2 |A(kappa) u - b|^2 and r = A(kappa) u - b
+//   dL/du       = A(kappa) r
+//   dL/dkappa_x = (P_x r) . (G_x P_x u)
+// and for the vector-Jacobian product of v = A(kappa) u with a cotangent w
+//   grad_u = A(kappa) w,   grad_kappa_x = (P_x w) . (G_x P_x u).
+//
+// One fused kernel per element (no quadrature-point data goes to HBM): the
+// gradient of u, F = G P u kept in registers, the divergence of kappa F
+// (loss mode: r = A u - b and 1/2 |r|^2 per element), the gradient of r (or
+// w), dL/dkappa = P r . F, and the divergence of kappa G P r.  G and kappa
+// are read twice, the second time while the element's first read is still
+// in the L2 cache.  The phases are those of the transposing BK3 mapping
+// (bk3_phases.hpp).
+#include "bk3_phases.hpp"
+#include "bkx_launch.hpp"
+
+namespace bkx {
+namespace {
+
+enum AdjMode { kLoss = 0, kVjp = 1, kApply = 2 };
+
+template <int NM, int NQ>
+struct AdjShape {
+    static constexpr int nq2 = NQ * NQ;
+    static constexpr int epb = 256 / nq2 > 0 ? 256 / nq2 : 1;
+    static constexpr int threads = ((epb * nq2 + kWave - 1) / kWave) * kWave;
+};
+
+template <int NM, int NQ, int MODE>
+__global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
+    size_t nelmt, const double* __restrict__ Bm, const double* __restrict__ Dm, const double* __restrict__ G,
+    const double* __restrict__ kappa, const double* __restrict__ u, const double* __restrict__ bw,
+    double* __restrict__ out_u, double* __restrict__ out_k, double* __restrict__ loss)
+{
+    using S = AdjShape<NM, NQ>;
+    using E = bk3p::Elem<NM, NQ>;
+    constexpr int nq2 = S::nq2;
+    __shared__ double s_box[S::epb][3 * E::box];
+
+    const int slot = threadIdx.x / nq2;
+    const int l = threadIdx.x - slot * nq2;
+    const bool in_elem = slot < S::epb;
+    const size_t e = size_t(blockIdx.x) * S::epb + slot;
+    const bool active = in_elem && e < nelmt;
+    const size_t ee = active ? e : 0;
+    E el;
+    el.init(s_box[in_elem ? slot : 0], l, in_elem);
+    const bool modal = in_elem && l < E::nm2;   // lane (j,k) of the modal plane
+
+    // gradient of u
+    double x[NM];
+#pragma unroll
+    for (int i = 0; i < NM; ++i) {
+        x[i] = (active && modal) ? u[ee * E::nm3 + i * E::nm2 + l] : 0.0;
+    }
+    double qr[NQ];
+    el.template grad<0>(Bm, Dm, x, qr);
+
+    // F = G P u (kept), kappa F -> rr (registers), rs -> B, rt -> C
+    const double* ge = G + ee * 6 * E::nq3 + l;      // G(e, f, p, q, r) at f = p = 0
+    const double* ke = kappa + ee * E::nq3 + l;      // kappa(e, p, q, r) at p = 0
+    double F0[NQ], F1[NQ], F2[NQ], rr[NQ];
+    if (in_elem) {
+#pragma unroll
+        for (int p = 0; p < NQ; ++p) {
+            double g[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                g[f] = active ? ge[f * E::nq3 + p * nq2] : 0.0;
+            }
+            const double kap = active ? ke[p * nq2] : 0.0;
+            const int xx = E::X(p, el.lq, el.lq2);
+            const double qs = el.B[xx];
+            const double qt = el.C[xx];
+            F0[p] = g[0] * qr[p] + g[1] * qs + g[2] * qt;
+            F1[p] = g[1] * qr[p] + g[3] * qs + g[4] * qt;
+            F2[p] = g[2] * qr[p] + g[4] * qs + g[5] * qt;
+            rr[p] = kap * F0[p];
+            el.B[xx] = kap * F1[p];
+            el.C[xx] = kap * F2[p];
+        }
+    }
+    __syncthreads();
+
+    double r[NM];
+    if constexpr (MODE == kVjp) {
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            r[i] = (active && modal) ? bw[ee * E::nm3 + i * E::nm2 + l] : 0.0;
+        }
+    } else {
+        double o[NM];
+        el.template div<10>(Bm, Dm, rr, o);
+        if constexpr (MODE == kApply) {
+            if (active && modal) {
+#pragma unroll
+                for (int i = 0; i < NM; ++i) {
+                    out_u[e * E::nm3 + i * E::nm2 + l] = o[i];
+                }
+            }
+            return;
+        } else {
+            double part = 0.0;
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                r[i] = (active && modal) ? o[i] - bw[ee * E::nm3 + i * E::nm2 + l] : 0.0;
+                part += r[i] * r[i];
+            }
+            // 1/2 |r_e|^2: the element's lanes summed in lane order through box C
+            // (free until the third phase of the next gradient)
+            if (modal) {
+                el.C[l] = part;
+            }
+            __syncthreads();
+            if (active && l == 0) {
+                double s = 0.0;
+                for (int t = 0; t < E::nm2; ++t) {
+                    s += el.C[t];
+                }
+                loss[e] = 0.5 * s;
+            }
+            __syncthreads();
+        }
+    }
+
+    // gradient of r, dL/dkappa = P r . F, kappa G P r
+    double qrr[NQ];
+    el.template grad<20>(Bm, Dm, r, qrr);
+    if (in_elem) {
+#pragma unroll
+        for (int p = 0; p < NQ; ++p) {
+            double g[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                g[f] = active ? ge[f * E::nq3 + p * nq2] : 0.0;
+            }
+            const double kap = active ? ke[p * nq2] : 0.0;
+            const int xx = E::X(p, el.lq, el.lq2);
+            const double qs = el.B[xx];
+            const double qt = el.C[xx];
+            if (active) {
+                out_k[e * E::nq3 + p * nq2 + l] = qrr[p] * F0[p] + qs * F1[p] + qt * F2[p];
+            }
+            rr[p] = kap * (g[0] * qrr[p] + g[1] * qs + g[2] * qt);
+            el.B[xx] = kap * (g[1] * qrr[p] + g[3] * qs + g[4] * qt);
+            el.C[xx] = kap * (g[2] * qrr[p] + g[4] * qs + g[5] * qt);
+        }
+    }
+    __syncthreads();
+    double o2[NM];
+    el.template div<30>(Bm, Dm, rr, o2);
+    if (active && modal) {
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            out_u[e * E::nm3 + i * E::nm2 + l] = o2[i];
+        }
+    }
+}
+
+template <int NM, int MODE>
+int launch_adj(size_t nelmt, const double* B, const double* D, const double* G, const double* kappa, const double* u,
+               const double* bw, double* out_u, double* out_k, double* loss, hipStream_t s)
+{
+    using S = AdjShape<NM, NM + 1>;
+    const unsigned grid = grid_for(nelmt, S::epb);
+    if (grid == 0) {
+        return BKX_ERROR_SIZE;
+    }
+    hipLaunchKernelGGL((bk3_adj_kernel<NM, NM + 1, MODE>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, G, kappa,
+                       u, bw, out_u, out_k, loss);
+    return launch_status();
+}
+
+// deterministic sum of n doubles into *out: one workgroup, fixed order
+constexpr int kSumThreads = 1024;
+
+__global__ __launch_bounds__(kSumThreads) void sum_kernel(const double* __restrict__ x, size_t n,
+                                                          double* __restrict__ out)
+{
+    __shared__ double part[kSumThreads];
+    double s = 0.0;
+    for (size_t i = threadIdx.x; i < n; i += kSumThreads) {
+        s += x[i];
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kSumThreads / 2; w > 0; w /= 2) {
+        if (int(threadIdx.x) < w) {
+            part[threadIdx.x] += part[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        *out = part[0];
+    }
+}
+
+}  // namespace
+
+int bk3_apply_kappa_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
+                        const double* u, double* v, hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto P) {
+        return launch_adj<decltype(P)::value + 1, kApply>(nelmt, B, D, G, kappa, u, nullptr, v, nullptr, nullptr, s);
+    });
+}
+
+int bk3_adjoint_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
+                    const double* u, const double* b, double* dldu, double* dldk, double* loss, hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto P) {
+        return launch_adj<decltype(P)::value + 1, kLoss>(nelmt, B, D, G, kappa, u, b, dldu, dldk, loss, s);
+    });
+}
+
+int bk3_vjp_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
+                const double* u, const double* w, double* gu, double* gk, hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto P) {
+        return launch_adj<decltype(P)::value + 1, kVjp>(nelmt, B, D, G, kappa, u, w, gu, gk, nullptr, s);
+    });
+}
+
+int sum_f64(const double* x, size_t n, double* out, hipStream_t s)
+{
+    hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kSumThreads), 0, s, x, n, out);
+    return launch_status();
+}
+
+}  // namespace bkx

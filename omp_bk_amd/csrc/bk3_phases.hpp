@@ -1,0 +1,261 @@
+// bk3_phases.hpp -- the phases of the transposing BK3 mapping (bk13t.hip) as
+// device functions, for kernels that chain several gradient / divergence
+// passes per element (the adjoint in bk3_adj.hip).
+//
+// An element owns nq^2 consecutive lanes of the workgroup (lane l) and three
+// padded LDS boxes A, B, C of nq^3 values indexed X(p|i, q|j, r|k).  Between
+// calls the workgroup must be synchronised by the caller where noted.
+//
+//   grad(x):  modal values x[i] of lane (j,k) (l < nm^2), column i
+//             -> qr[p] in lane (q,r), column p; qs(p,q,r) in box B and
+//                qt(p,q,r) in box C (the reference gradient at the Gauss
+//                points, openmp/bk3_sumfact.h:63-119)
+//   div(rr):  rr[p] in lane (q,r) column p, rs in box B, rt in box C
+//             -> o[i] in lane (j,k) (l < nm^2), column i:
+//                o = B^T-interpolated transposed gradient
+//                (openmp/bk3_sumfact.h:126-177)
+#pragma once
+
+#include "bkx_device.hpp"
+
+namespace bkx {
+namespace bk3p {
+
+template <bool SB>
+__device__ __forceinline__ void row_fence()
+{
+    if constexpr (SB) {
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// out[c] = sum_m M[m * LD + c] in[m]
+template <int LD, int NIN, int NOUT, typename P>
+__device__ __forceinline__ void cols(P M, const double (&in)[NIN], double (&out)[NOUT])
+{
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) {
+        out[c] = 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < NIN; ++m) {
+#pragma unroll
+        for (int c = 0; c < NOUT; ++c) {
+            out[c] += M[m * LD + c] * in[m];
+        }
+    }
+}
+
+template <int LD, int NIN, int NOUT, typename P>
+__device__ __forceinline__ void cols2(P M, const double (&in0)[NIN], double (&out0)[NOUT],
+                                      const double (&in1)[NIN], double (&out1)[NOUT])
+{
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) {
+        out0[c] = 0.0;
+        out1[c] = 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < NIN; ++m) {
+#pragma unroll
+        for (int c = 0; c < NOUT; ++c) {
+            out0[c] += M[m * LD + c] * in0[m];
+            out1[c] += M[m * LD + c] * in1[m];
+        }
+    }
+}
+
+// out[c] = sum_m M[c * LD + m] in[m]
+template <int LD, int NIN, int NOUT, typename P>
+__device__ __forceinline__ void rows(P M, const double (&in)[NIN], double (&out)[NOUT])
+{
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) {
+        double t = 0.0;
+#pragma unroll
+        for (int m = 0; m < NIN; ++m) {
+            t += M[c * LD + m] * in[m];
+        }
+        out[c] = t;
+    }
+}
+
+template <int LD, int NIN, int NOUT, typename P>
+__device__ __forceinline__ void rows3(P M, const double (&in0)[NIN], double (&out0)[NOUT],
+                                      const double (&in1)[NIN], double (&out1)[NOUT],
+                                      const double (&in2)[NIN], double (&out2)[NOUT])
+{
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) {
+        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+#pragma unroll
+        for (int m = 0; m < NIN; ++m) {
+            t0 += M[c * LD + m] * in0[m];
+            t1 += M[c * LD + m] * in1[m];
+            t2 += M[c * LD + m] * in2[m];
+        }
+        out0[c] = t0;
+        out1[c] = t1;
+        out2[c] = t2;
+    }
+}
+
+template <int NM, int NQ>
+struct Elem {
+    static constexpr int nm2 = NM * NM;
+    static constexpr int nm3 = nm2 * NM;
+    static constexpr int nq2 = NQ * NQ;
+    static constexpr int nq3 = nq2 * NQ;
+    static constexpr int row = NQ + 1;
+    static constexpr int box = NQ * NQ * row;
+
+    double* A;
+    double* B;
+    double* C;
+    int l;            // lane within the element's plane, 0 .. nq^2 - 1
+    bool in_elem;     // lane belongs to an element slot of the workgroup
+    int lm, lm2;      // (l / NM, l % NM)
+    int lq, lq2;      // (l / NQ, l % NQ)
+
+    __device__ static int X(int a, int b, int c) { return (a * NQ + b) * row + c; }
+
+    __device__ void init(double* boxes, int lane, bool in)
+    {
+        A = boxes;
+        B = boxes + box;
+        C = boxes + 2 * box;
+        l = lane;
+        in_elem = in;
+        lm = l / NM;
+        lm2 = l - lm * NM;
+        lq = l / NQ;
+        lq2 = l - lq * NQ;
+    }
+
+    // modal x (lane (j,k), l < nm^2) -> qr (lane (q,r)), qs -> B, qt -> C.
+    // Ends with the workgroup synchronised.
+    template <int TAG>
+    __device__ void grad(const double* Bm, const double* Dm, const double (&x)[NM], double (&qr)[NQ])
+    {
+        if (in_elem && l < nm2) {
+            double a[NQ];
+            cols<NQ>(launder<TAG + 1>(Bm), x, a);
+#pragma unroll
+            for (int p = 0; p < NQ; ++p) {
+                A[X(p, lm, lm2)] = a[p];
+            }
+        }
+        __syncthreads();
+        if (in_elem && l < NQ * NM) {
+            double y[NM], b[NQ];
+#pragma unroll
+            for (int j = 0; j < NM; ++j) {
+                y[j] = A[X(lm, j, lm2)];
+            }
+            cols<NQ>(launder<TAG + 2>(Bm), y, b);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                B[X(lm, q, lm2)] = b[q];
+            }
+        }
+        __syncthreads();
+        if (in_elem) {
+            double y[NM], U[NQ], qt[NQ];
+#pragma unroll
+            for (int k = 0; k < NM; ++k) {
+                y[k] = B[X(lq, lq2, k)];
+            }
+            cols<NQ>(launder<TAG + 3>(Bm), y, U);
+            cols<NQ>(launder<TAG + 4>(Dm), U, qt);
+#pragma unroll
+            for (int r = 0; r < NQ; ++r) {
+                A[X(lq, lq2, r)] = U[r];
+                C[X(lq, lq2, r)] = qt[r];
+            }
+        }
+        __syncthreads();
+        if (in_elem) {
+            double y[NQ], z[NQ], qs[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                y[q] = A[X(lq, q, lq2)];
+                z[q] = A[X(q, lq, lq2)];
+            }
+            cols2<NQ>(launder<TAG + 5>(Dm), y, qs, z, qr);
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                B[X(lq, q, lq2)] = qs[q];
+            }
+        }
+        __syncthreads();
+    }
+
+    // rr (lane (q,r)), rs in B, rt in C -> o (lane (j,k), l < nm^2).
+    // Starts after a workgroup synchronisation (the caller's writes of B, C);
+    // ends with the workgroup synchronised.
+    template <int TAG>
+    __device__ void div(const double* Bm, const double* Dm, const double (&rr)[NQ], double (&o)[NM])
+    {
+        double Ws[NQ], Wt[NQ];
+        if (in_elem) {
+            double Wr[NQ], y[NQ], z[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                y[q] = B[X(lq, q, lq2)];
+                z[q] = C[X(lq, lq2, q)];
+            }
+            rows3<NQ>(launder<TAG + 1>(Dm), rr, Wr, y, Ws, z, Wt);
+#pragma unroll
+            for (int p = 0; p < NQ; ++p) {
+                A[X(p, lq, lq2)] = Wr[p];
+            }
+        }
+        __syncthreads();
+        if (in_elem) {
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                B[X(lq, q, lq2)] = Ws[q];
+            }
+        }
+        __syncthreads();
+        if (in_elem) {
+            double W[NQ], Y[NM];
+#pragma unroll
+            for (int r = 0; r < NQ; ++r) {
+                const int x = X(lq, lq2, r);
+                W[r] = Wt[r] + A[x] + B[x];
+            }
+            rows<NQ>(launder<TAG + 2>(Bm), W, Y);
+#pragma unroll
+            for (int k = 0; k < NM; ++k) {
+                A[X(lq, lq2, k)] = Y[k];
+            }
+        }
+        __syncthreads();
+        if (in_elem && l < NQ * NM) {
+            double y[NQ], Z[NM];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                y[q] = A[X(lm, q, lm2)];
+            }
+            rows<NQ>(launder<TAG + 3>(Bm), y, Z);
+#pragma unroll
+            for (int j = 0; j < NM; ++j) {
+                B[X(lm, j, lm2)] = Z[j];
+            }
+        }
+        __syncthreads();
+        if (in_elem && l < nm2) {
+            double z[NQ];
+#pragma unroll
+            for (int p = 0; p < NQ; ++p) {
+                z[p] = B[X(p, lm, lm2)];
+            }
+            rows<NQ>(launder<TAG + 4>(Bm), z, o);
+        }
+        __syncthreads();
+    }
+};
+
+}  // namespace bk3p
+}  // namespace bkx

@@ -22,6 +22,8 @@
 // HBM traffic per element: u once (s nq^3 B, s = sizeof storage), v once
 // (8 nq^3 B) and G once (6 s nq^3 B), all with unit-stride lanes: u/v planes
 // are nq^2 contiguous values, G(i, j, f, :) rows are nq contiguous values.
+#include <type_traits>
+
 #include "bkx_device.hpp"
 #include "bkx_launch.hpp"
 
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(kWave * WAVES) void bk5_kernel(size_t nelmt, const 
 // 3 nq^3 writes and 4 nq^4 reads when the j and k contractions read their
 // operands from LDS (the slab mapping above).  MINW is the occupancy
 // (wavefronts per SIMD) the register allocation is held to.
-template <int NQ, int WAVES, typename TS, bool NT, int MINW>
+template <int NQ, int WAVES, typename TS, bool NT, int MINW, bool GSTAGE, int PD>
 __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(MINW))) void bk5t_kernel(size_t nelmt, const double* __restrict__ D,
                                                             const TS* __restrict__ G, const TS* __restrict__ u,
                                                             double* __restrict__ v)
@@ -201,6 +203,15 @@ __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(M
 
     __shared__ double s_a[WAVES][epw][S::box];
     __shared__ double s_b[WAVES][epw][S::box];
+    // GSTAGE: one layer of G of every element of the wavefront, loaded with
+    // unit-stride 8-byte lanes and read back per lane (G(e, i, :, :, :) is
+    // 6 nq^2 contiguous values per element).
+    using stage_t = std::conditional_t<sizeof(TS) == 4, f32x2, double>;
+    constexpr int tpe = sizeof(TS) == 4 ? 2 : 1;            // values per 8-byte unit
+    constexpr int layer = 6 * nq2;                           // values per layer
+    constexpr int units = epw * layer / tpe;                 // 8-byte units per wavefront layer
+    constexpr int upl = (units + kWave - 1) / kWave;         // units per lane
+    __shared__ alignas(8) TS s_g[GSTAGE ? WAVES : 1][GSTAGE ? epw * layer : 1];
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -228,9 +239,33 @@ __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(M
     // first layer of G
     const TS* ge = G + ee * 6 * nq3 + a * 6 * NQ + b;
     TS gn[6];
+    stage_t gs[PD][upl];     // PD layers of G in flight
+    // the wavefront's elements are consecutive: its G is one contiguous run
+    const size_t e0 = (size_t(blockIdx.x) * WAVES + wave) * epw;
+    const int nact = int(nelmt - e0 < size_t(epw) ? nelmt - e0 : size_t(epw));
+    auto stage_load = [&](int c, stage_t (&dst)[upl]) {
 #pragma unroll
-    for (int f = 0; f < 6; ++f) {
-        gn[f] = active ? load<NT>(ge + f * NQ) : TS(0);
+        for (int t = 0; t < upl; ++t) {
+            const int x = t * kWave + lane;                  // unit within the wavefront's layer
+            const int el = x * tpe / layer;                  // element slot of the unit
+            const int off = x * tpe - el * layer;            // value offset within that layer
+            const bool ok = x < units && el < nact;
+            const TS* src = G + (e0 + (ok ? el : 0)) * 6 * nq3 + c * layer + (ok ? off : 0);
+            dst[t] = ok ? *reinterpret_cast<const stage_t*>(src) : stage_t{};
+        }
+    };
+    if constexpr (GSTAGE) {
+#pragma unroll
+        for (int d = 0; d < PD; ++d) {
+            if (d < NQ) {
+                stage_load(d, gs[d]);
+            }
+        }
+    } else {
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            gn[f] = active ? load<NT>(ge + f * NQ) : TS(0);
+        }
     }
     if (in_elem) {
 #pragma unroll
@@ -280,14 +315,34 @@ __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(M
 #pragma unroll
     for (int c = 0; c < NQ; ++c) {
         double g[6];
+        if constexpr (GSTAGE) {
+            TS* sg = s_g[wave];
 #pragma unroll
-        for (int f = 0; f < 6; ++f) {
-            g[f] = double(gn[f]);
-        }
-        if (c + 1 < NQ) {
+            for (int t = 0; t < upl; ++t) {
+                const int x = t * kWave + lane;
+                if (x < units) {
+                    *reinterpret_cast<stage_t*>(sg + x * tpe) = gs[c % PD][t];
+                }
+            }
+            if (c + PD < NQ) {
+                stage_load(c + PD, gs[c % PD]);
+            }
+            wave_sync();
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                gn[f] = active ? load<NT>(ge + (c + 1) * 6 * nq2 + f * NQ) : TS(0);
+                g[f] = double(sg[sl * layer + (a * 6 + f) * NQ + b]);
+            }
+            wave_sync();
+        } else {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                g[f] = double(gn[f]);
+            }
+            if (c + 1 < NQ) {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    gn[f] = active ? load<NT>(ge + (c + 1) * 6 * nq2 + f * NQ) : TS(0);
+                }
             }
         }
         const double s_ = sb[X(c, a, b)];
@@ -361,7 +416,7 @@ __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(M
     }
 }
 
-template <int NQ, int WAVES, typename TS, bool NT, int MINW = 1>
+template <int NQ, int WAVES, typename TS, bool NT, int MINW = 1, bool GSTAGE = false, int PD = 1>
 int launch_bk5t_v(size_t nelmt, const double* D, const TS* G, const TS* u, double* v, hipStream_t s)
 {
     constexpr size_t per_block = size_t(WAVES) * Bk5Shape<NQ>::epw;
@@ -369,7 +424,7 @@ int launch_bk5t_v(size_t nelmt, const double* D, const TS* G, const TS* u, doubl
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk5t_kernel<NQ, WAVES, TS, NT, MINW>), dim3(grid), dim3(kWave * WAVES), 0, s, nelmt, D, G, u, v);
+    hipLaunchKernelGGL((bk5t_kernel<NQ, WAVES, TS, NT, MINW, GSTAGE, PD>), dim3(grid), dim3(kWave * WAVES), 0, s, nelmt, D, G, u, v);
     return launch_status();
 }
 
@@ -386,16 +441,21 @@ int launch_bk5_v(size_t nelmt, const double* D, const TS* G, const TS* u, double
 }
 
 // Launch variants (the autotuner's search space): the slab mapping (0-3) or
-// the transposing mapping (4-8), wavefronts per workgroup, ordinary or
-// non-temporal loads, and an occupancy target.  Ordinary loads come first:
+// the transposing mapping (4-11), wavefronts per workgroup, ordinary or
+// non-temporal loads, an occupancy target, and G read per lane or staged
+// through LDS by unit-stride loads (9-15) with 1 to 4 layers in flight.  Ordinary loads come first:
 // on MI355X the non-temporal hint costs 10% (p = 7) to 40% (p = 3), as a
 // G row of one factor is 64 contiguous bytes and the next factor's row
 // fills the rest of the same 128-byte line.
-constexpr int kBk5Variants = 9;
+constexpr int kBk5Variants = 16;
 
 template <int NQ, typename TS>
 int launch_bk5(int variant, size_t nelmt, const double* D, const TS* G, const TS* u, double* v, hipStream_t s)
 {
+    // the staged variants read G in 8-byte units: they need an 8-byte-aligned G
+    if (variant >= 9 && reinterpret_cast<uintptr_t>(G) % 8 != 0) {
+        variant = 4;
+    }
     switch (variant) {
     case 0:
         return launch_bk5_v<NQ, 4, TS, false>(nelmt, D, G, u, v, s);
@@ -415,6 +475,20 @@ int launch_bk5(int variant, size_t nelmt, const double* D, const TS* G, const TS
         return launch_bk5t_v<NQ, 1, TS, true>(nelmt, D, G, u, v, s);
     case 8:
         return launch_bk5t_v<NQ, 4, TS, false, 3>(nelmt, D, G, u, v, s);
+    case 9:
+        return launch_bk5t_v<NQ, 2, TS, false, 1, true>(nelmt, D, G, u, v, s);
+    case 10:
+        return launch_bk5t_v<NQ, 1, TS, false, 1, true>(nelmt, D, G, u, v, s);
+    case 11:
+        return launch_bk5t_v<NQ, 4, TS, false, 1, true>(nelmt, D, G, u, v, s);
+    case 12:
+        return launch_bk5t_v<NQ, 1, TS, false, 1, true, 2>(nelmt, D, G, u, v, s);
+    case 13:
+        return launch_bk5t_v<NQ, 1, TS, false, 1, true, 3>(nelmt, D, G, u, v, s);
+    case 14:
+        return launch_bk5t_v<NQ, 2, TS, false, 1, true, 2>(nelmt, D, G, u, v, s);
+    case 15:
+        return launch_bk5t_v<NQ, 1, TS, false, 1, true, 4>(nelmt, D, G, u, v, s);
     default:
         return BKX_ERROR_ARG;
     }

@@ -12,6 +12,17 @@ int bk1_apply_f64(int p, size_t nelmt, const double* B, const double* JxW, const
                   hipStream_t s);
 int bk3_apply_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* u,
                   double* v, hipStream_t s);
+int bk1t_apply_f64(int variant, int p, size_t nelmt, const double* B, const double* JxW, const double* u,
+                   double* v, hipStream_t s);
+int bk3t_apply_f64(int variant, int p, size_t nelmt, const double* B, const double* D, const double* G,
+                   const double* u, double* v, hipStream_t s);
+int bk3_apply_kappa_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
+                        const double* u, double* v, hipStream_t s);
+int bk3_adjoint_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
+                    const double* u, const double* b, double* dldu, double* dldk, double* loss, hipStream_t s);
+int bk3_vjp_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
+                const double* u, const double* w, double* gu, double* gk, hipStream_t s);
+int sum_f64(const double* x, size_t n, double* out, hipStream_t s);
 int bk5_variants();
 int bk5_apply_f64(int variant, int p, size_t nelmt, const double* D, const double* G, const double* u, double* v,
                   hipStream_t s);
@@ -74,6 +85,11 @@ int bkx_tune_count(int op, int form)
     if (op == 5 && (form == BKX_FORM_F64 || form == BKX_FORM_F32W)) {
         return bkx::bk5_variants();
     }
+    if ((op == 1 || op == 3) && form == BKX_FORM_F64) {
+        // 0-2: transposing mapping (bk13t.hip) with all, half or a third of
+        // the elements per workgroup; 3: LDS-contraction mapping (bk13_f64.hip)
+        return 4;
+    }
     return (op == 1 || op == 3 || op == 5) && form >= 0 && form < kForms ? 1 : -1;
 }
 
@@ -116,7 +132,11 @@ int bkx_bk1_apply_f64(int p, size_t nelmt, const double* basis, const double* Jx
     if (!basis || !JxW || !in || !out) {
         return BKX_ERROR_NULL;
     }
-    return bkx::bk1_apply_f64(p, nelmt, basis, JxW, in, out, as_stream(stream));
+    const int var = variant_of(1, BKX_FORM_F64, p);
+    if (var == 3) {
+        return bkx::bk1_apply_f64(p, nelmt, basis, JxW, in, out, as_stream(stream));
+    }
+    return bkx::bk1t_apply_f64(var, p, nelmt, basis, JxW, in, out, as_stream(stream));
 }
 
 int bkx_bk3_apply_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
@@ -131,7 +151,11 @@ int bkx_bk3_apply_f64(int p, size_t nelmt, const double* basis, const double* db
     if (!basis || !dbasis || !G || !in || !out) {
         return BKX_ERROR_NULL;
     }
-    return bkx::bk3_apply_f64(p, nelmt, basis, dbasis, G, in, out, as_stream(stream));
+    const int var = variant_of(3, BKX_FORM_F64, p);
+    if (var == 3) {
+        return bkx::bk3_apply_f64(p, nelmt, basis, dbasis, G, in, out, as_stream(stream));
+    }
+    return bkx::bk3t_apply_f64(var, p, nelmt, basis, dbasis, G, in, out, as_stream(stream));
 }
 
 int bkx_bk5_apply_f64(int p, size_t nelmt, const double* dbasis, const double* G, const double* in, double* out,
@@ -177,6 +201,61 @@ int bkx_bk5_apply_f32w(int p, size_t nelmt, const double* dbasis, const float* G
         return BKX_ERROR_NULL;
     }
     return bkx::bk5_apply_f32w(variant_of(5, BKX_FORM_F32W, p), p, nelmt, dbasis, G, in, out, as_stream(stream));
+}
+
+int bkx_bk3_apply_kappa_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                            const double* kappa, const double* in, double* out, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !kappa || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_apply_kappa_f64(p, nelmt, basis, dbasis, G, kappa, in, out, as_stream(stream));
+}
+
+int bkx_bk3_adjoint_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                        const double* kappa, const double* u, const double* b, double* dldu, double* dldkappa,
+                        double* loss, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !kappa || !u || !b || !dldu || !dldkappa || !loss) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_adjoint_f64(p, nelmt, basis, dbasis, G, kappa, u, b, dldu, dldkappa, loss, as_stream(stream));
+}
+
+int bkx_bk3_vjp_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                    const double* kappa, const double* u, const double* w, double* grad_u, double* grad_kappa,
+                    bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !kappa || !u || !w || !grad_u || !grad_kappa) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_vjp_f64(p, nelmt, basis, dbasis, G, kappa, u, w, grad_u, grad_kappa, as_stream(stream));
+}
+
+int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream)
+{
+    if (!out || (n > 0 && !x)) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::sum_f64(x, n, out, as_stream(stream));
 }
 
 }  // extern "C"

@@ -13,6 +13,7 @@ constexpr int kWave = 64;   // CDNA wavefront width
 // Native 16-byte vectors (the nontemporal builtins take clang vector types).
 using f64x2 = double __attribute__((ext_vector_type(2)));
 using f32x4 = float __attribute__((ext_vector_type(4)));
+using f32x2 = float __attribute__((ext_vector_type(2)));
 
 // Orders LDS traffic between the lanes of one wavefront.  A wavefront's LDS
 // instructions are executed in issue order, so a lane may read what another

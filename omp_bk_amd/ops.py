@@ -113,3 +113,61 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
     with torch.cuda.device(u.device):
         _lib.call(entry, p, ne, dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
     return out
+
+
+def _bk3_common(p, basis, dbasis, G, u):
+    _order_ok(3, p)
+    nm, nq = p + 1, p + 2
+    if u.numel() % nm ** 3:
+        raise ValueError("u must hold whole elements of %d values" % nm ** 3)
+    ne = u.numel() // nm ** 3
+    _check("basis", basis, nm * nq)
+    _check("dbasis", dbasis, nq * nq)
+    _check("G", G, ne * 6 * nq ** 3)
+    _check("u", u, ne * nm ** 3)
+    return ne, nm, nq
+
+
+def bk3_kappa(p, basis, dbasis, G, kappa, u, out=None):
+    """v = A(kappa) u: BK3 with each Gauss point's six factors scaled by
+    kappa(e, p, q, r) (FP64)."""
+    ne, nm, nq = _bk3_common(p, basis, dbasis, G, u)
+    _check("kappa", kappa, ne * nq ** 3)
+    if out is None:
+        out = torch.empty_like(u)
+    _check("out", out, ne * nm ** 3)
+    with torch.cuda.device(u.device):
+        _lib.call("bkx_bk3_apply_kappa_f64", p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(),
+                  kappa.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
+    return out
+
+
+def bk3_adjoint(p, basis, dbasis, G, kappa, u, b):
+    """For L = 1/2 |A(kappa) u - b|^2: returns (L, dL/du, dL/dkappa, loss per
+    element), L as a 0-d device tensor summed in a fixed order."""
+    ne, nm, nq = _bk3_common(p, basis, dbasis, G, u)
+    _check("kappa", kappa, ne * nq ** 3)
+    _check("b", b, ne * nm ** 3)
+    dldu = torch.empty_like(u)
+    dldk = torch.empty_like(kappa)
+    loss_e = torch.empty(ne, dtype=torch.float64, device=u.device)
+    L = torch.empty((), dtype=torch.float64, device=u.device)
+    with torch.cuda.device(u.device):
+        _lib.call("bkx_bk3_adjoint_f64", p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), kappa.data_ptr(),
+                  u.data_ptr(), b.data_ptr(), dldu.data_ptr(), dldk.data_ptr(), loss_e.data_ptr(), _stream(u))
+        _lib.call("bkx_sum_f64", loss_e.data_ptr(), ne, L.data_ptr(), _stream(u))
+    return L, dldu, dldk, loss_e
+
+
+def bk3_vjp(p, basis, dbasis, G, kappa, u, w):
+    """Vector-Jacobian product of v = A(kappa) u with cotangent w:
+    (A(kappa) w, (grad w) . G (grad u))."""
+    ne, nm, nq = _bk3_common(p, basis, dbasis, G, u)
+    _check("kappa", kappa, ne * nq ** 3)
+    _check("w", w, ne * nm ** 3)
+    gu = torch.empty_like(u)
+    gk = torch.empty_like(kappa)
+    with torch.cuda.device(u.device):
+        _lib.call("bkx_bk3_vjp_f64", p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), kappa.data_ptr(),
+                  u.data_ptr(), w.data_ptr(), gu.data_ptr(), gk.data_ptr(), _stream(u))
+    return gu, gk

@@ -1,0 +1,128 @@
+"""BK3 with a coefficient kappa and its hand-written adjoint, on the device,
+against the FP64 oracle (bko_bk3_adjoint: the operator with kappa, the
+residual, dL/du = A(kappa) r and dL/dkappa = grad r . G grad u), against
+central finite differences of the device loss, and through the
+torch.autograd wrapper (gradcheck).  Workload: p = 6, E = 16384, kappa
+log-normal (sigma = 0.5), u and b uniform[-1, 1]."""
+import numpy as np
+import pytest
+import torch
+
+import _oracle as O
+from omp_bk_amd import autograd
+from omp_bk_amd import geometry as geo
+from omp_bk_amd import ops
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _host_case(p, E, seed):
+    rng = np.random.default_rng(seed)
+    B, D = geo.bk13_bases(p)
+    shape = geo.box_shape(E) if (E & (E - 1)) == 0 else (E, 1, 1)
+    G = geo.geometric_factors(3, p, E, shape=shape).reshape(-1)
+    nm, nq = p + 1, p + 2
+    kappa = np.exp(0.5 * rng.standard_normal(E * nq ** 3))
+    u = rng.uniform(-1, 1, E * nm ** 3)
+    b = rng.uniform(-1, 1, E * nm ** 3)
+    return B, D, G, kappa, u, b
+
+
+def _t(x):
+    return torch.as_tensor(np.ascontiguousarray(x), device=DEV)
+
+
+@pytest.mark.parametrize("p", range(1, 9))
+@pytest.mark.parametrize("E", [1, 7])
+def test_against_oracle(p, E):
+    B, D, G, kappa, u, b = _host_case(p, E, seed=10 * p + E)
+    nq = p + 2
+    Lr, r, dldu_r, dldk_r = O.bk3_adjoint(nq, B, D, G, kappa, u, b)
+    v = ops.bk3_kappa(p, _t(B), _t(D), _t(G), _t(kappa), _t(u)).cpu().numpy()
+    assert O.rel_maxdiff(v, O.bk3_kappa(nq, B, D, G, kappa, u)) <= 1e-13
+    L, dldu, dldk, loss_e = ops.bk3_adjoint(p, _t(B), _t(D), _t(G), _t(kappa), _t(u), _t(b))
+    assert abs(L.item() - Lr) <= 1e-13 * Lr
+    assert O.rel_maxdiff(dldu.cpu().numpy(), dldu_r) <= 1e-13
+    assert O.rel_maxdiff(dldk.cpu().numpy(), dldk_r) <= 1e-13
+    # VJP with cotangent w: b' = A u - w makes the residual w
+    w = np.random.default_rng(p).uniform(-1, 1, u.size)
+    _, _, gu_r, gk_r = O.bk3_adjoint(nq, B, D, G, kappa, u, O.bk3_kappa(nq, B, D, G, kappa, u) - w)
+    gu, gk = ops.bk3_vjp(p, _t(B), _t(D), _t(G), _t(kappa), _t(u), _t(w))
+    assert O.rel_maxdiff(gu.cpu().numpy(), gu_r) <= 1e-13
+    assert O.rel_maxdiff(gk.cpu().numpy(), gk_r) <= 1e-12
+
+
+def _device_case(p, E, seed):
+    g = torch.Generator(device=DEV).manual_seed(seed)
+    B, D = (torch.as_tensor(a, device=DEV) for a in geo.bk13_bases(p))
+    G = geo.geometric_factors(3, p, E, device=DEV).reshape(-1)
+    nm, nq = p + 1, p + 2
+    kappa = torch.exp(0.5 * torch.randn(E * nq ** 3, dtype=torch.float64, device=DEV, generator=g))
+    u = torch.rand(E * nm ** 3, dtype=torch.float64, device=DEV, generator=g) * 2 - 1
+    b = torch.rand(E * nm ** 3, dtype=torch.float64, device=DEV, generator=g) * 2 - 1
+    return B, D, G, kappa, u, b, g
+
+
+def test_full_size_finite_differences():
+    """p = 6, E = 16384: central differences (h = 1e-6) of the device loss
+    in 64 random directions each for u and kappa.  L is quadratic in u and
+    in kappa separately, so the central difference is exact up to rounding;
+    the per-element losses of the two evaluations are differenced before
+    they are summed, which keeps that rounding near eps |L_e| sqrt(E) / h."""
+    p, E = 6, 16384
+    B, D, G, kappa, u, b, g = _device_case(p, E, 7)
+    L, dldu, dldk, loss_e = ops.bk3_adjoint(p, B, D, G, kappa, u, b)
+    full = 0.5 * torch.sum((ops.bk3_kappa(p, B, D, G, kappa, u) - b) ** 2).item()
+    assert abs(full - L.item()) <= 1e-12 * L.item()
+    per_elem = lambda uu, kk: ops.bk3_adjoint(p, B, D, G, kk, uu, b)[3]
+    h = 1e-6
+    for t in range(64):
+        du = torch.randn(u.shape, dtype=torch.float64, device=DEV, generator=g)
+        dk = torch.randn(kappa.shape, dtype=torch.float64, device=DEV, generator=g)
+        fd_u = torch.sum(per_elem(u + h * du, kappa) - per_elem(u - h * du, kappa)).item() / (2 * h)
+        fd_k = torch.sum(per_elem(u, kappa + h * dk) - per_elem(u, kappa - h * dk)).item() / (2 * h)
+        an_u = torch.dot(dldu, du).item()
+        an_k = torch.dot(dldk, dk).item()
+        assert abs(fd_u - an_u) <= 1e-6 * abs(an_u), (t, fd_u, an_u)
+        assert abs(fd_k - an_k) <= 1e-6 * abs(an_k), (t, fd_k, an_k)
+
+
+def test_full_size_sampled_oracle():
+    """p = 6, E = 16384: a sample of elements against the FP64 oracle."""
+    p, E = 6, 16384
+    B, D, G, kappa, u, b, _ = _device_case(p, E, 11)
+    L, dldu, dldk, loss_e = ops.bk3_adjoint(p, B, D, G, kappa, u, b)
+    nm, nq = p + 1, p + 2
+    idx = torch.as_tensor(np.random.default_rng(0).choice(E, 128, replace=False), device=DEV)
+    h = lambda t, n: t.view(E, n)[idx].cpu().numpy()
+    Lr, _, dldu_r, dldk_r = O.bk3_adjoint(nq, B.cpu().numpy(), D.cpu().numpy(), h(G, 6 * nq ** 3),
+                                          h(kappa, nq ** 3), h(u, nm ** 3), h(b, nm ** 3))
+    assert O.rel_maxdiff(h(dldu, nm ** 3).reshape(-1), dldu_r) <= 1e-13
+    assert O.rel_maxdiff(h(dldk, nq ** 3).reshape(-1), dldk_r) <= 1e-13
+    assert abs(loss_e[idx].sum().item() - Lr) <= 1e-13 * Lr
+
+
+def test_autograd_gradcheck():
+    p, E = 3, 3
+    B, D, G, kappa, u, b, _ = _device_case(p, 8, 5)
+    nm, nq = p + 1, p + 2
+    u = u[: E * nm ** 3].clone().requires_grad_(True)
+    kappa = kappa[: E * nq ** 3].clone().requires_grad_(True)
+    G = G[: E * 6 * nq ** 3].contiguous()
+    f = lambda uu, kk: autograd.bk3_kappa(uu, kk, p, B, D, G)
+    assert torch.autograd.gradcheck(f, (u, kappa), eps=1e-6, atol=1e-7, rtol=1e-6)
+
+
+def test_autograd_loss_matches_adjoint():
+    p, E = 6, 64
+    B, D, G, kappa, u, b, _ = _device_case(p, E, 9)
+    uu = u.clone().requires_grad_(True)
+    kk = kappa.clone().requires_grad_(True)
+    v = autograd.bk3_kappa(uu, kk, p, B, D, G)
+    L = 0.5 * torch.sum((v - b) ** 2)
+    L.backward()
+    La, dldu, dldk, _ = ops.bk3_adjoint(p, B, D, G, kappa, u, b)
+    assert abs(L.item() - La.item()) <= 1e-13 * La.item()
+    assert ((uu.grad - dldu).abs().max() / dldu.abs().max()).item() <= 1e-13
+    assert ((kk.grad - dldk).abs().max() / dldk.abs().max()).item() <= 1e-13

@@ -1,0 +1,66 @@
+"""Every launch variant of every operator form against the CPU oracle
+(the variants are what the autotuner chooses between, so each must be
+correct on its own): ragged element counts, LCG data, <= 1e-13."""
+import numpy as np
+import pytest
+import torch
+
+import _oracle as O
+from omp_bk_amd import _lib
+from omp_bk_amd import ops
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+FORM = {"f64": 0, "f32w": 2}
+
+
+def _dev(x, dt=torch.float64):
+    return torch.as_tensor(np.ascontiguousarray(x), device=DEV).to(dt)
+
+
+def _cases():
+    out = []
+    for op, forms, orders in ((1, ["f64"], range(1, 9)), (3, ["f64"], range(1, 9)), (5, ["f64", "f32w"], range(1, 8))):
+        for form in forms:
+            for v in range(_lib.lib().bkx_tune_count(op, FORM[form])):
+                for p in orders:
+                    out.append((op, form, v, p))
+    return out
+
+
+@pytest.mark.parametrize("op,form,variant,p", _cases())
+def test_variant(op, form, variant, p):
+    L = _lib.lib()
+    f = FORM[form]
+    E = 13
+    _lib.call("bkx_tune_set", op, f, p, variant)
+    try:
+        if op == 5:
+            n = p + 1
+            u, G = O.fill_random(E * n ** 3, E * 6 * n ** 3, seed=variant * 100 + p)
+            D = O.test_basis(n, n)
+            if form == "f32w":
+                G = G.astype(np.float32).astype(np.float64)
+            dt = torch.float64 if form == "f64" else torch.float32
+            out = torch.full((u.size,), float("nan"), dtype=torch.float64, device=DEV)
+            ops.bk5(p, _dev(D), _dev(G, dt), _dev(u, dt), out=out, form=form)
+            ref = O.bk5(n, D, G, u)
+        else:
+            nm, nq = p + 1, p + 2
+            B = O.test_basis(nm, nq)
+            Dm = O.test_basis(nq, nq)
+            nf = 1 if op == 1 else 6
+            u, Q = O.fill_random(E * nm ** 3, E * nf * nq ** 3, seed=variant * 100 + p)
+            out = torch.full((u.size,), float("nan"), dtype=torch.float64, device=DEV)
+            if op == 1:
+                ops.bk1(p, _dev(B), _dev(Q), _dev(u), out=out)
+                ref = O.bk1(nq, B, Q, u)
+            else:
+                ops.bk3(p, _dev(B), _dev(Dm), _dev(Q), _dev(u), out=out)
+                ref = O.bk3(nq, B, Dm, Q, u)
+        got = out.cpu().numpy()
+        assert np.all(np.isfinite(got))
+        assert O.rel_maxdiff(got, ref) <= 1e-13
+    finally:
+        _lib.call("bkx_tune_set", op, f, p, 0)
+        assert L.bkx_tune_get(op, f, p) == 0
