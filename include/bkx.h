@@ -144,6 +144,51 @@ int bkx_bk3_vjp_f64(int p, size_t nelmt, const double* basis, const double* dbas
 int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
+/* Mixed-order BK3 batches (no reference counterpart: the reference runs    */
+/* one order per binary, include/bk_common.h:143-178).                     */
+/*                                                                          */
+/* A batch holds nelmt elements in the caller's order, element e of order   */
+/* p_e in 1..8 (n = p + 1, q = p + 2): its n^3 values of u / v start at     */
+/* uoff[e] = sum_{e' < e} n_e'^3 and its 6 q^3 factors of G (factor-major,  */
+/* as bkx_bk3_apply_f64) at goff[e] = sum_{e' < e} 6 q_e'^3.  `bases` packs */
+/* B_p (n x q) then D_p (q x q) for p = 1, 2, ..., 8 in turn.               */
+/* ---------------------------------------------------------------------- */
+
+typedef struct bkx_mixed_plan bkx_mixed_plan;
+typedef struct bkx_graph bkx_graph;
+
+/* Builds a plan (host work plus a few small device arrays): the elements
+ * sorted by order and cut into tiles of rounds_by_order[p] x epb elements,
+ * epb = floor(256 / q^2) / div_by_order[p] (div 1 or 2; either array may be
+ * NULL for 1), kept both grouped by order and sorted by decreasing cost.
+ * grid <= 0 picks the number of resident workgroups of the device. */
+int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
+                          const int32_t* rounds_by_order, int grid, bkx_mixed_plan** plan);
+int bkx_mixed_plan_destroy(bkx_mixed_plan* plan);
+/* Sizes of u/v (ndof) and G (ngeo) in doubles, the tile count and grid. */
+int bkx_mixed_plan_info(const bkx_mixed_plan* plan, size_t* ndof, size_t* ngeo, int* ntiles, int* grid);
+/* The element offsets uoff, goff (host arrays of nelmt). */
+int bkx_mixed_plan_offsets(const bkx_mixed_plan* plan, int64_t* uoff, int64_t* goff);
+
+/* One launch over every tile: one workgroup per tile (default), or
+ * persistent (workgroup g takes tiles g, g + grid, ...) when the plan was
+ * made with rounds_by_order or a grid > 0. */
+int bkx_bk3_mixed_apply_f64(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
+                            double* out, bkx_stream_t stream);
+/* One launch per order (the comparison). */
+int bkx_bk3_mixed_apply_by_order_f64(const bkx_mixed_plan* plan, const double* bases, const double* G,
+                                     const double* in, double* out, bkx_stream_t stream);
+/* The launch of one order's tiles only (used by the tuner). */
+int bkx_bk3_mixed_apply_order_f64(const bkx_mixed_plan* plan, int p, const double* bases, const double* G,
+                                  const double* in, double* out, bkx_stream_t stream);
+/* The persistent launch (by_order = 0) or the per-order launches captured
+ * in a hipGraph for the given arrays; replay with bkx_graph_launch. */
+int bkx_bk3_mixed_graph_create(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
+                               double* out, int by_order, bkx_graph** graph);
+int bkx_graph_launch(bkx_graph* graph, bkx_stream_t stream);
+int bkx_graph_destroy(bkx_graph* graph);
+
+/* ---------------------------------------------------------------------- */
 /* Bandwidth probes (the roofs the operators are measured against)         */
 /* ---------------------------------------------------------------------- */
 

@@ -1,0 +1,504 @@
+// bk3_mixed.hip -- BK3 over a batch of elements of mixed order (p = 1..8)
+// in FP64 on gfx950, as one persistent launch or as one launch per order.
+//
+// The reference runs one order per binary (include/bk_common.h:143-178); a
+// batch of elements of different orders has no counterpart there.  Layout of
+// a mixed batch, in the caller's element order e = 0 .. nelmt-1 with order
+// p_e (n_e = p_e + 1 modes, q_e = p_e + 2 points per direction):
+//   u, v:  element e's n_e^3 values at uoff[e] = sum_{e' < e} n_e'^3
+//   G:     element e's 6 q_e^3 factors (factor-major, as BK3) at
+//          goff[e] = sum_{e' < e} 6 q_e'^3
+//   bases: for p = 1..8 in turn, B_p (n x q) then D_p (q x q)
+// The plan (host side) sorts the elements by order, cuts each order's list
+// into tiles of rounds x epb elements (epb = elements per workgroup of that
+// order's kernel body, chosen per order by the tuner) and keeps two tile
+// lists on the device: grouped by order (for one launch per order) and
+// sorted by decreasing cost (for the persistent launch, where workgroup g
+// takes tiles g, g + grid, ...: a static longest-first assignment, no
+// atomics, no counters to reset).  Every tile runs the transposing BK3 body
+// of bk13t_body.hpp for its order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <new>
+#include <numeric>
+#include <vector>
+
+#include "bk13t_body.hpp"
+#include "bkx.h"
+#include "bkx_launch.hpp"
+
+namespace bkx {
+namespace {
+
+struct MixedTile {
+    int32_t p;       // order
+    int32_t div;     // elements-per-workgroup divisor of that order's body (1 or 2)
+    int32_t first;   // first index into the order-sorted element list
+    int32_t count;   // elements in the tile
+};
+
+constexpr int kMixedThreads = 256;
+
+template <int P>
+constexpr int smem_of()
+{
+    using S = bk13t::Bk13tShape<P + 1, P + 2, 1>;
+    return S::epb * 3 * S::box;
+}
+
+template <int P>
+constexpr int max_smem()
+{
+    if constexpr (P == 1) {
+        return smem_of<1>();
+    } else {
+        return smem_of<P>() > max_smem<P - 1>() ? smem_of<P>() : max_smem<P - 1>();
+    }
+}
+
+constexpr int kMixedSmem = max_smem<8>();   // doubles
+
+template <int P, int DIV, bool LOOP>
+__device__ __forceinline__ void run_tile(int tid, const MixedTile& t, const int32_t* __restrict__ elems,
+                                         const int64_t* __restrict__ uoff, const int64_t* __restrict__ goff,
+                                         const double* __restrict__ bases, const int64_t* __restrict__ boff,
+                                         const double* __restrict__ G, const double* __restrict__ u,
+                                         double* __restrict__ v, double* smem)
+{
+    constexpr int NM = P + 1;
+    constexpr int NQ = P + 2;
+    using S = bk13t::Bk13tShape<NM, NQ, DIV>;
+    const double* B = bases + boff[P];
+    const double* D = B + NM * NQ;
+    const int slot = tid / S::nq2;
+    const int l = tid - slot * S::nq2;
+    if constexpr (!LOOP) {
+        // one round: the tile holds at most epb elements
+        const bool active = slot < S::epb && slot < t.count;
+        const int e = elems[t.first + (active ? slot : 0)];
+        bk13t::bk13t_body<NM, NQ, true, DIV, false>(B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l,
+                                                    smem);
+    } else {
+        for (int r0 = 0; r0 < t.count; r0 += S::epb) {
+            const int idx = r0 + slot;
+            const bool active = slot < S::epb && idx < t.count;
+            const int e = elems[t.first + (active ? idx : 0)];
+            bk13t::bk13t_body<NM, NQ, true, DIV, false>(B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l,
+                                                        smem, t.first + r0);
+            __syncthreads();
+        }
+    }
+}
+
+template <int DIV, bool LOOP>
+__device__ __forceinline__ void run_tile_p(int tid, const MixedTile& t, const int32_t* elems, const int64_t* uoff,
+                                           const int64_t* goff, const double* bases, const int64_t* boff,
+                                           const double* G, const double* u, double* v, double* smem)
+{
+    switch (t.p) {
+    case 1:
+        run_tile<1, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        break;
+    case 2:
+        run_tile<2, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        break;
+    case 3:
+        run_tile<3, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        break;
+    case 4:
+        run_tile<4, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        break;
+    case 5:
+        run_tile<5, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        break;
+    case 6:
+        run_tile<6, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        break;
+    case 7:
+        run_tile<7, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        break;
+    default:
+        run_tile<8, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        break;
+    }
+}
+
+// LOOP = false: one workgroup per tile (a single launch over all orders; the
+// hardware dispatches the workgroups, longest tiles first, as CUs free up).
+// LOOP = true: persistent, workgroup g processes tiles g, g + gridDim.x, ...
+// In the persistent form the basis rows are read with vector loads (loads
+// after the loop's barriers and stores cannot be proven unclobbered, so they
+// do not become scalar loads), which costs registers and occupancy.
+template <bool LOOP>
+__global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
+    int ntiles, const MixedTile* __restrict__ tiles, const int32_t* __restrict__ elems,
+    const int64_t* __restrict__ uoff, const int64_t* __restrict__ goff, const double* __restrict__ bases,
+    const int64_t* __restrict__ boff, const double* __restrict__ G, const double* __restrict__ u,
+    double* __restrict__ v)
+{
+    __shared__ double smem[kMixedSmem];
+    auto one = [&](int k, int tid) {
+        const MixedTile t = tiles[k];
+        if (t.div == 2) {
+            run_tile_p<2, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        } else {
+            run_tile_p<1, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        }
+    };
+    if constexpr (LOOP) {
+        for (int k = blockIdx.x; k < ntiles; k += gridDim.x) {
+            // The lane index, made to depend on the tile: the bodies' lane
+            // arithmetic is then computed in the body that runs instead of
+            // being hoisted out of the loop for all of them.
+            int tid = threadIdx.x;
+            asm("; tile %1" : "+v"(tid) : "s"(k));
+            one(k, tid);
+        }
+    } else {
+        if (int(blockIdx.x) < ntiles) {
+            one(blockIdx.x, threadIdx.x);
+        }
+    }
+}
+
+int n_of(int p) { return p + 1; }
+int q_of(int p) { return p + 2; }
+
+}  // namespace
+}  // namespace bkx
+
+struct bkx_mixed_plan {
+    size_t nelmt = 0;
+    size_t ndof = 0;
+    size_t ngeo = 0;
+    int ntiles = 0;
+    int grid = 0;
+    bool persistent = false;    // loop over tiles (grid resident workgroups) or one workgroup per tile
+    int tile_begin[10] = {};            // per order: range of the grouped tile list
+    std::vector<int64_t> uoff_h, goff_h;
+    bkx::MixedTile* tiles_grouped = nullptr;
+    bkx::MixedTile* tiles_persist = nullptr;
+    int32_t* elems = nullptr;
+    int64_t* uoff = nullptr;
+    int64_t* goff = nullptr;
+    int64_t* boff = nullptr;
+};
+
+struct bkx_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+template <typename T>
+int upload(T** dst, const T* src, size_t n)
+{
+    if (hipMalloc(reinterpret_cast<void**>(dst), std::max<size_t>(1, n) * sizeof(T)) != hipSuccess) {
+        *dst = nullptr;
+        return BKX_ERROR_DEVICE;
+    }
+    if (n > 0 && hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+        return BKX_ERROR_DEVICE;
+    }
+    return BKX_SUCCESS;
+}
+
+int launch_tiles(const bkx::MixedTile* tiles, int ntiles, bool persistent, const bkx_mixed_plan* pl,
+                 const double* bases, const double* G, const double* in, double* out, hipStream_t s)
+{
+    if (ntiles == 0) {
+        return BKX_SUCCESS;
+    }
+    if (persistent) {
+        hipLaunchKernelGGL(bkx::bk3_mixed_kernel<true>, dim3(std::min(pl->grid, ntiles)), dim3(bkx::kMixedThreads), 0,
+                           s, ntiles, tiles, pl->elems, pl->uoff, pl->goff, bases, pl->boff, G, in, out);
+    } else {
+        hipLaunchKernelGGL(bkx::bk3_mixed_kernel<false>, dim3(ntiles), dim3(bkx::kMixedThreads), 0, s, ntiles, tiles,
+                           pl->elems, pl->uoff, pl->goff, bases, pl->boff, G, in, out);
+    }
+    return bkx::launch_status();
+}
+
+}  // namespace
+
+extern "C" {
+
+int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
+                          const int32_t* rounds_by_order, int grid, bkx_mixed_plan** plan)
+{
+    if (!plan || (nelmt > 0 && !order)) {
+        return BKX_ERROR_NULL;
+    }
+    *plan = nullptr;
+    if (nelmt > size_t(INT32_MAX)) {
+        return BKX_ERROR_SIZE;
+    }
+    auto* pl = new (std::nothrow) bkx_mixed_plan;
+    if (!pl) {
+        return BKX_ERROR_DEVICE;
+    }
+    pl->nelmt = nelmt;
+    pl->uoff_h.resize(nelmt);
+    pl->goff_h.resize(nelmt);
+    std::vector<std::vector<int32_t>> by_p(9);
+    int64_t uo = 0, go = 0;
+    for (size_t e = 0; e < nelmt; ++e) {
+        const int p = order[e];
+        if (p < 1 || p > 8) {
+            delete pl;
+            return BKX_ERROR_ORDER;
+        }
+        pl->uoff_h[e] = uo;
+        pl->goff_h[e] = go;
+        uo += int64_t(bkx::n_of(p)) * bkx::n_of(p) * bkx::n_of(p);
+        go += int64_t(6) * bkx::q_of(p) * bkx::q_of(p) * bkx::q_of(p);
+        by_p[p].push_back(int32_t(e));
+    }
+    pl->ndof = size_t(uo);
+    pl->ngeo = size_t(go);
+
+    std::vector<int32_t> elems;
+    std::vector<bkx::MixedTile> grouped;
+    std::vector<double> cost;
+    for (int p = 1; p <= 8; ++p) {
+        const int q = bkx::q_of(p);
+        const int div = (div_by_order && div_by_order[p] == 2) ? 2 : 1;
+        const int full = std::max(1, 256 / (q * q));
+        const int epb = std::max(1, full / div);
+        const int rounds = (rounds_by_order && rounds_by_order[p] > 0) ? rounds_by_order[p] : 1;
+        if (rounds > 1) {
+            pl->persistent = true;
+        }
+        const int per_tile = epb * rounds;
+        pl->tile_begin[p] = int(grouped.size());
+        const int first = int(elems.size());
+        elems.insert(elems.end(), by_p[p].begin(), by_p[p].end());
+        const int cnt = int(by_p[p].size());
+        for (int t = 0; t < cnt; t += per_tile) {
+            const int c = std::min(per_tile, cnt - t);
+            grouped.push_back(bkx::MixedTile{p, div, first + t, c});
+            const int n = bkx::n_of(p);
+            cost.push_back(double(c) * (2.0 * n * n * n + 6.0 * q * q * q));
+        }
+    }
+    pl->tile_begin[9] = int(grouped.size());
+    pl->ntiles = int(grouped.size());
+    std::vector<int> idx(grouped.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<bkx::MixedTile> persist(grouped.size());
+    for (size_t k = 0; k < idx.size(); ++k) {
+        persist[k] = grouped[idx[k]];
+    }
+    int64_t boff[10] = {};
+    int64_t bo = 0;
+    for (int p = 1; p <= 8; ++p) {
+        boff[p] = bo;
+        bo += int64_t(bkx::n_of(p)) * bkx::q_of(p) + int64_t(bkx::q_of(p)) * bkx::q_of(p);
+    }
+    boff[9] = bo;
+
+    if (grid <= 0) {
+        int dev = 0, cus = 256, per_cu = 1;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        }
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bkx::bk3_mixed_kernel<true>, bkx::kMixedThreads,
+                                                         0) !=
+                hipSuccess ||
+            per_cu < 1) {
+            per_cu = 1;
+        }
+        grid = cus * per_cu;
+    }
+    if (grid > 0 && rounds_by_order == nullptr) {
+        pl->persistent = true;
+    }
+    pl->grid = std::max(1, std::min(grid, std::max(1, pl->ntiles)));
+
+    int st = BKX_SUCCESS;
+    st = st ? st : upload(&pl->tiles_grouped, grouped.data(), grouped.size());
+    st = st ? st : upload(&pl->tiles_persist, persist.data(), persist.size());
+    st = st ? st : upload(&pl->elems, elems.data(), elems.size());
+    st = st ? st : upload(&pl->uoff, pl->uoff_h.data(), nelmt);
+    st = st ? st : upload(&pl->goff, pl->goff_h.data(), nelmt);
+    st = st ? st : upload(&pl->boff, boff, 10);
+    if (st != BKX_SUCCESS) {
+        bkx_mixed_plan_destroy(pl);
+        return st;
+    }
+    *plan = pl;
+    return BKX_SUCCESS;
+}
+
+int bkx_mixed_plan_destroy(bkx_mixed_plan* plan)
+{
+    if (!plan) {
+        return BKX_SUCCESS;
+    }
+    for (void* ptr : {static_cast<void*>(plan->tiles_grouped), static_cast<void*>(plan->tiles_persist),
+                      static_cast<void*>(plan->elems), static_cast<void*>(plan->uoff),
+                      static_cast<void*>(plan->goff), static_cast<void*>(plan->boff)}) {
+        if (ptr) {
+            (void)hipFree(ptr);
+        }
+    }
+    delete plan;
+    return BKX_SUCCESS;
+}
+
+int bkx_mixed_plan_info(const bkx_mixed_plan* plan, size_t* ndof, size_t* ngeo, int* ntiles, int* grid)
+{
+    if (!plan) {
+        return BKX_ERROR_NULL;
+    }
+    if (ndof) {
+        *ndof = plan->ndof;
+    }
+    if (ngeo) {
+        *ngeo = plan->ngeo;
+    }
+    if (ntiles) {
+        *ntiles = plan->ntiles;
+    }
+    if (grid) {
+        *grid = plan->grid;
+    }
+    return BKX_SUCCESS;
+}
+
+int bkx_mixed_plan_offsets(const bkx_mixed_plan* plan, int64_t* uoff, int64_t* goff)
+{
+    if (!plan || !uoff || !goff) {
+        return BKX_ERROR_NULL;
+    }
+    std::copy(plan->uoff_h.begin(), plan->uoff_h.end(), uoff);
+    std::copy(plan->goff_h.begin(), plan->goff_h.end(), goff);
+    return BKX_SUCCESS;
+}
+
+int bkx_bk3_mixed_apply_f64(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
+                            double* out, bkx_stream_t stream)
+{
+    if (!plan) {
+        return BKX_ERROR_NULL;
+    }
+    if (plan->nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!bases || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return launch_tiles(plan->tiles_persist, plan->ntiles, plan->persistent, plan, bases, G, in, out,
+                        reinterpret_cast<hipStream_t>(stream));
+}
+
+int bkx_bk3_mixed_apply_by_order_f64(const bkx_mixed_plan* plan, const double* bases, const double* G,
+                                     const double* in, double* out, bkx_stream_t stream)
+{
+    if (!plan) {
+        return BKX_ERROR_NULL;
+    }
+    if (plan->nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!bases || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    for (int p = 1; p <= 8; ++p) {
+        const int n = plan->tile_begin[p + 1] - plan->tile_begin[p];
+        const int st = launch_tiles(plan->tiles_grouped + plan->tile_begin[p], n, false, plan, bases, G, in, out,
+                                    reinterpret_cast<hipStream_t>(stream));
+        if (st != BKX_SUCCESS) {
+            return st;
+        }
+    }
+    return BKX_SUCCESS;
+}
+
+int bkx_bk3_mixed_apply_order_f64(const bkx_mixed_plan* plan, int p, const double* bases, const double* G,
+                                  const double* in, double* out, bkx_stream_t stream)
+{
+    if (!plan) {
+        return BKX_ERROR_NULL;
+    }
+    if (p < 1 || p > 8) {
+        return BKX_ERROR_ORDER;
+    }
+    if (!bases || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    const int n = plan->tile_begin[p + 1] - plan->tile_begin[p];
+    return launch_tiles(plan->tiles_grouped + plan->tile_begin[p], n, false, plan, bases, G, in, out,
+                        reinterpret_cast<hipStream_t>(stream));
+}
+
+int bkx_bk3_mixed_graph_create(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
+                               double* out, int by_order, bkx_graph** graph)
+{
+    if (!plan || !graph) {
+        return BKX_ERROR_NULL;
+    }
+    *graph = nullptr;
+    auto* g = new (std::nothrow) bkx_graph;
+    if (!g) {
+        return BKX_ERROR_DEVICE;
+    }
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        delete g;
+        return BKX_ERROR_DEVICE;
+    }
+    int st = BKX_SUCCESS;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        st = BKX_ERROR_DEVICE;
+    } else {
+        st = by_order ? bkx_bk3_mixed_apply_by_order_f64(plan, bases, G, in, out, s)
+                      : bkx_bk3_mixed_apply_f64(plan, bases, G, in, out, s);
+        hipGraph_t captured = nullptr;
+        const hipError_t end = hipStreamEndCapture(s, &captured);
+        if (st == BKX_SUCCESS && end != hipSuccess) {
+            st = BKX_ERROR_DEVICE;
+        }
+        g->graph = captured;
+        if (st == BKX_SUCCESS && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+            st = BKX_ERROR_DEVICE;
+        }
+    }
+    (void)hipStreamDestroy(s);
+    if (st != BKX_SUCCESS) {
+        bkx_graph_destroy(g);
+        return st;
+    }
+    *graph = g;
+    return BKX_SUCCESS;
+}
+
+int bkx_graph_launch(bkx_graph* graph, bkx_stream_t stream)
+{
+    if (!graph || !graph->exec) {
+        return BKX_ERROR_NULL;
+    }
+    return hipGraphLaunch(graph->exec, reinterpret_cast<hipStream_t>(stream)) == hipSuccess ? BKX_SUCCESS
+                                                                                           : BKX_ERROR_LAUNCH;
+}
+
+int bkx_graph_destroy(bkx_graph* graph)
+{
+    if (!graph) {
+        return BKX_SUCCESS;
+    }
+    if (graph->exec) {
+        (void)hipGraphExecDestroy(graph->exec);
+    }
+    if (graph->graph) {
+        (void)hipGraphDestroy(graph->graph);
+    }
+    delete graph;
+    return BKX_SUCCESS;
+}
+
+}  // extern "C"

@@ -238,8 +238,8 @@ __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(M
     }
     // first layer of G
     const TS* ge = G + ee * 6 * nq3 + a * 6 * NQ + b;
-    TS gn[6];
-    stage_t gs[PD][upl];     // PD layers of G in flight
+    TS gn[PD][6];            // PD layers of G in flight (per-lane reads)
+    stage_t gs[PD][upl];     // PD layers of G in flight (staged reads)
     // the wavefront's elements are consecutive: its G is one contiguous run
     const size_t e0 = (size_t(blockIdx.x) * WAVES + wave) * epw;
     const int nact = int(nelmt - e0 < size_t(epw) ? nelmt - e0 : size_t(epw));
@@ -263,8 +263,13 @@ __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(M
         }
     } else {
 #pragma unroll
-        for (int f = 0; f < 6; ++f) {
-            gn[f] = active ? load<NT>(ge + f * NQ) : TS(0);
+        for (int d = 0; d < PD; ++d) {
+            if (d < NQ) {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    gn[d][f] = active ? load<NT>(ge + d * 6 * nq2 + f * NQ) : TS(0);
+                }
+            }
         }
     }
     if (in_elem) {
@@ -336,12 +341,12 @@ __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(M
         } else {
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                g[f] = double(gn[f]);
+                g[f] = double(gn[c % PD][f]);
             }
-            if (c + 1 < NQ) {
+            if (c + PD < NQ) {
 #pragma unroll
                 for (int f = 0; f < 6; ++f) {
-                    gn[f] = active ? load<NT>(ge + (c + 1) * 6 * nq2 + f * NQ) : TS(0);
+                    gn[c % PD][f] = active ? load<NT>(ge + (c + PD) * 6 * nq2 + f * NQ) : TS(0);
                 }
             }
         }
@@ -442,12 +447,12 @@ int launch_bk5_v(size_t nelmt, const double* D, const TS* G, const TS* u, double
 
 // Launch variants (the autotuner's search space): the slab mapping (0-3) or
 // the transposing mapping (4-11), wavefronts per workgroup, ordinary or
-// non-temporal loads, an occupancy target, and G read per lane or staged
-// through LDS by unit-stride loads (9-15) with 1 to 4 layers in flight.  Ordinary loads come first:
+// non-temporal loads, an occupancy target, G read per lane or staged
+// through LDS by unit-stride loads (9-18), and 1 to 8 layers of G in flight.  Ordinary loads come first:
 // on MI355X the non-temporal hint costs 10% (p = 7) to 40% (p = 3), as a
 // G row of one factor is 64 contiguous bytes and the next factor's row
 // fills the rest of the same 128-byte line.
-constexpr int kBk5Variants = 16;
+constexpr int kBk5Variants = 22;
 
 template <int NQ, typename TS>
 int launch_bk5(int variant, size_t nelmt, const double* D, const TS* G, const TS* u, double* v, hipStream_t s)
@@ -489,6 +494,18 @@ int launch_bk5(int variant, size_t nelmt, const double* D, const TS* G, const TS
         return launch_bk5t_v<NQ, 2, TS, false, 1, true, 2>(nelmt, D, G, u, v, s);
     case 15:
         return launch_bk5t_v<NQ, 1, TS, false, 1, true, 4>(nelmt, D, G, u, v, s);
+    case 16:
+        return launch_bk5t_v<NQ, 1, TS, false, 1, true, 6>(nelmt, D, G, u, v, s);
+    case 17:
+        return launch_bk5t_v<NQ, 1, TS, false, 1, true, 8>(nelmt, D, G, u, v, s);
+    case 18:
+        return launch_bk5t_v<NQ, 2, TS, false, 1, true, 4>(nelmt, D, G, u, v, s);
+    case 19:
+        return launch_bk5t_v<NQ, 1, TS, false, 1, false, 2>(nelmt, D, G, u, v, s);
+    case 20:
+        return launch_bk5t_v<NQ, 1, TS, false, 1, false, 3>(nelmt, D, G, u, v, s);
+    case 21:
+        return launch_bk5t_v<NQ, 2, TS, false, 1, false, 2>(nelmt, D, G, u, v, s);
     default:
         return BKX_ERROR_ARG;
     }

@@ -38,12 +38,40 @@ bool order_ok(int op, int p) { return p >= bkx_min_order(op) && p <= bkx_max_ord
 
 hipStream_t as_stream(bkx_stream_t s) { return reinterpret_cast<hipStream_t>(s); }
 
-// Launch variant of each (operator, form, order), set by the autotuner;
-// variant 0 until then.
+// Launch variant of each (operator, form, order).  The defaults are the
+// fastest variants measured on an MI355X by omp_bk_amd/tune.py (BK5 over
+// E = 65536, BK3 over 16384, BK1 over 32768 elements); the autotuner
+// overwrites them at run time.
 constexpr int kOps = 6;
 constexpr int kForms = 4;
 constexpr int kOrders = 9;
-std::atomic<int> g_variant[kOps][kForms][kOrders];
+
+struct VariantTable {
+    std::atomic<int> v[kOps][kForms][kOrders];
+    VariantTable()
+    {
+        for (auto& op : v) {
+            for (auto& form : op) {
+                for (auto& x : form) {
+                    x.store(0, std::memory_order_relaxed);
+                }
+            }
+        }
+        //                            p:   0   1   2   3   4   5   6   7   8
+        static const int bk5_f64[kOrders] = {0, 20, 1, 21, 15, 17, 20, 4, 0};
+        static const int bk5_f32w[kOrders] = {0, 17, 17, 17, 17, 17, 17, 17, 0};
+        static const int bk3_f64[kOrders] = {0, 1, 1, 1, 1, 1, 1, 1, 1};
+        static const int bk1_f64[kOrders] = {0, 3, 3, 3, 3, 1, 1, 1, 1};
+        for (int p = 0; p < kOrders; ++p) {
+            v[5][BKX_FORM_F64][p].store(bk5_f64[p], std::memory_order_relaxed);
+            v[5][BKX_FORM_F32W][p].store(bk5_f32w[p], std::memory_order_relaxed);
+            v[3][BKX_FORM_F64][p].store(bk3_f64[p], std::memory_order_relaxed);
+            v[1][BKX_FORM_F64][p].store(bk1_f64[p], std::memory_order_relaxed);
+        }
+    }
+};
+VariantTable g_table;
+auto& g_variant = g_table.v;
 
 bool tune_index_ok(int op, int form, int p)
 {

@@ -47,10 +47,12 @@ __device__ __forceinline__ void pin(float& x) { asm volatile("" : "+v"(x)); }
 template <typename T>
 using global_ptr = const T __attribute__((address_space(1)))*;
 
+// dep (a wavefront-uniform value) makes the launder depend on it, so that
+// in a loop the loads are not hoisted out when dep is the loop counter.
 template <int TAG, typename T>
-__device__ __forceinline__ global_ptr<T> launder(const T* p)
+__device__ __forceinline__ global_ptr<T> launder(const T* p, int dep = 0)
 {
-    asm("; launder %1" : "+s"(p) : "i"(TAG));
+    asm("; launder %1 %2" : "+s"(p) : "i"(TAG), "s"(dep));
     const uint64_t x = reinterpret_cast<uint64_t>(p);
     const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(x));
     const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(x >> 32));

@@ -1,0 +1,230 @@
+"""Mixed-order BK3 batches: workload, plan, launches and tuning.
+
+A batch holds elements of orders p = 1..8 in one E-vector (layout in
+include/bkx.h: element e's values at uoff[e], its factors at goff[e], the
+bases of all orders packed).  The workload of the benchmark: 200,000
+elements whose orders follow a Zipf(s = 1.2) distribution over a seeded
+shuffled list of the eight orders, each element with its own
+perturbed-affine geometry (x = x0 + A_e xi, A_e = h (I + a R_e), R_e uniform
+in [-1, 1]^{3x3}) and FP64 uniform[-1, 1] inputs.
+
+The plan cuts each order's elements into tiles; ``tune`` picks, per order,
+the elements per workgroup (divisor 1 or 2 of floor(256 / q^2)) by timing
+that order's launch, then the batch runs as one launch over all tiles
+(optionally captured in a hipGraph) or, for comparison, one launch per order.
+"""
+import ctypes
+import statistics
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import geometry as geo
+
+_vp = ctypes.c_void_p
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+def _bind():
+    L = _lib.lib()
+    if getattr(L, "_bkx_mixed_bound", False):
+        return L
+    L.bkx_mixed_plan_create.argtypes = [_i32p, ctypes.c_size_t, _i32p, _i32p, ctypes.c_int, ctypes.POINTER(_vp)]
+    L.bkx_mixed_plan_destroy.argtypes = [_vp]
+    L.bkx_mixed_plan_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.bkx_mixed_plan_offsets.argtypes = [_vp, _i64p, _i64p]
+    L.bkx_bk3_mixed_apply_f64.argtypes = [_vp] + [_vp] * 5
+    L.bkx_bk3_mixed_apply_by_order_f64.argtypes = [_vp] + [_vp] * 5
+    L.bkx_bk3_mixed_apply_order_f64.argtypes = [_vp, ctypes.c_int] + [_vp] * 5
+    L.bkx_bk3_mixed_graph_create.argtypes = [_vp] + [_vp] * 4 + [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.bkx_graph_launch.argtypes = [_vp, _vp]
+    L.bkx_graph_destroy.argtypes = [_vp]
+    for name in ("bkx_mixed_plan_create", "bkx_mixed_plan_destroy", "bkx_mixed_plan_info", "bkx_mixed_plan_offsets",
+                 "bkx_bk3_mixed_apply_f64", "bkx_bk3_mixed_apply_by_order_f64", "bkx_bk3_mixed_apply_order_f64",
+                 "bkx_bk3_mixed_graph_create", "bkx_graph_launch", "bkx_graph_destroy"):
+        getattr(L, name).restype = ctypes.c_int
+    L._bkx_mixed_bound = True
+    return L
+
+
+def _check(name, status):
+    if status != 0:
+        raise _lib.BkxError(name, status, _lib.lib().bkx_status_string(status).decode())
+
+
+# ---------------------------------------------------------------------------
+# workload
+# ---------------------------------------------------------------------------
+
+def zipf_orders(nelmt=200000, s=1.2, seed=2024):
+    """Orders of nelmt elements: rank k = 1..8 of a seeded shuffled list of
+    the orders 1..8 gets the Zipf(s) share k^-s / sum_j j^-s of the elements
+    (rounded, the remainder to rank 1), and the element order is shuffled."""
+    rng = np.random.default_rng(seed)
+    ranked = rng.permutation(np.arange(1, 9))
+    w = np.arange(1, 9, dtype=np.float64) ** -s
+    w /= w.sum()
+    counts = np.floor(w * nelmt).astype(np.int64)
+    counts[0] += nelmt - counts.sum()
+    order = np.repeat(ranked, counts).astype(np.int32)
+    rng.shuffle(order)
+    return order
+
+
+def packed_bases():
+    """B_p (n x q) then D_p (q x q) for p = 1..8 (the layout of include/bkx.h)."""
+    parts = []
+    for p in range(1, 9):
+        B, D = geo.bk13_bases(p)
+        parts += [B.reshape(-1), D.reshape(-1)]
+    return np.concatenate(parts)
+
+
+def affine_factors(order, goff, ngeo, amp=0.1, seed=7, device=None):
+    """G of every element of a mixed batch: element e is x = x0 + A_e xi with
+    A_e = h (I + amp R_e), h = 1 / cbrt(nelmt); G = w detJ J^-1 J^-T (J = A_e,
+    constant over the element) at its Gauss points, factor-major."""
+    nelmt = order.size
+    rng = np.random.default_rng(seed)
+    h = 1.0 / np.cbrt(nelmt)
+    A = h * (np.eye(3)[None] + amp * rng.uniform(-1, 1, (nelmt, 3, 3)))
+    inv = np.linalg.inv(A)
+    M = np.einsum("eab,ecb->eac", inv, inv) * np.linalg.det(A)[:, None, None]
+    sym = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    G = np.empty(ngeo)
+    for p in range(1, 9):
+        idx = np.nonzero(order == p)[0]
+        if idx.size == 0:
+            continue
+        q = p + 2
+        _, w = geo.gauss(q)
+        w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
+        vals = np.stack([M[idx, a, b] for a, b in sym], axis=1)[:, :, None] * w3[None, None, :]   # (k, 6, q^3)
+        pos = goff[idx][:, None] + np.arange(6 * q ** 3)[None, :]
+        G[pos] = vals.reshape(idx.size, -1)
+    return G if device is None else torch.as_tensor(G, device=device)
+
+
+# ---------------------------------------------------------------------------
+# plan
+# ---------------------------------------------------------------------------
+
+class MixedPlan:
+    """The device-side tile lists of one batch (bkx_mixed_plan_create)."""
+
+    def __init__(self, order, div_by_order=None, rounds_by_order=None, grid=0):
+        L = _bind()
+        self.order = np.ascontiguousarray(order, dtype=np.int32)
+        self.nelmt = self.order.size
+        div = None if div_by_order is None else np.ascontiguousarray(div_by_order, dtype=np.int32)
+        rnd = None if rounds_by_order is None else np.ascontiguousarray(rounds_by_order, dtype=np.int32)
+        self._div, self._rnd = div, rnd
+        h = _vp()
+        _check("bkx_mixed_plan_create", L.bkx_mixed_plan_create(
+            self.order.ctypes.data_as(_i32p), self.nelmt, None if div is None else div.ctypes.data_as(_i32p),
+            None if rnd is None else rnd.ctypes.data_as(_i32p), int(grid), ctypes.byref(h)))
+        self.handle = h
+        nd, ng, nt, gr = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int(), ctypes.c_int()
+        _check("bkx_mixed_plan_info", L.bkx_mixed_plan_info(h, ctypes.byref(nd), ctypes.byref(ng), ctypes.byref(nt),
+                                                            ctypes.byref(gr)))
+        self.ndof, self.ngeo, self.ntiles, self.grid = nd.value, ng.value, nt.value, gr.value
+        self.uoff = np.empty(self.nelmt, dtype=np.int64)
+        self.goff = np.empty(self.nelmt, dtype=np.int64)
+        _check("bkx_mixed_plan_offsets", L.bkx_mixed_plan_offsets(h, self.uoff.ctypes.data_as(_i64p),
+                                                                  self.goff.ctypes.data_as(_i64p)))
+
+    def close(self):
+        if self.handle:
+            _bind().bkx_mixed_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _args(self, bases, G, u, v):
+        for t in (bases, G, u, v):
+            assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
+        assert u.numel() == self.ndof and v.numel() == self.ndof and G.numel() == self.ngeo
+        return bases.data_ptr(), G.data_ptr(), u.data_ptr(), v.data_ptr()
+
+    def apply(self, bases, G, u, v):
+        """One launch over all tiles, on the current torch stream."""
+        _check("bkx_bk3_mixed_apply_f64", _bind().bkx_bk3_mixed_apply_f64(
+            self.handle, *self._args(bases, G, u, v), torch.cuda.current_stream().cuda_stream))
+        return v
+
+    def apply_by_order(self, bases, G, u, v):
+        """One launch per order."""
+        _check("bkx_bk3_mixed_apply_by_order_f64", _bind().bkx_bk3_mixed_apply_by_order_f64(
+            self.handle, *self._args(bases, G, u, v), torch.cuda.current_stream().cuda_stream))
+        return v
+
+    def apply_order(self, p, bases, G, u, v):
+        _check("bkx_bk3_mixed_apply_order_f64", _bind().bkx_bk3_mixed_apply_order_f64(
+            self.handle, p, *self._args(bases, G, u, v), torch.cuda.current_stream().cuda_stream))
+        return v
+
+    def graph(self, bases, G, u, v, by_order=False):
+        """A hipGraph of the launch(es) for these arrays (MixedGraph)."""
+        return MixedGraph(self, bases, G, u, v, by_order)
+
+
+class MixedGraph:
+    def __init__(self, plan, bases, G, u, v, by_order):
+        h = _vp()
+        _check("bkx_bk3_mixed_graph_create", _bind().bkx_bk3_mixed_graph_create(
+            plan.handle, *plan._args(bases, G, u, v), int(by_order), ctypes.byref(h)))
+        self.handle = h
+        self._keep = (plan, bases, G, u, v)
+
+    def launch(self):
+        _check("bkx_graph_launch", _bind().bkx_graph_launch(self.handle, torch.cuda.current_stream().cuda_stream))
+
+    def close(self):
+        if self.handle:
+            _bind().bkx_graph_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _median_ms(fn, steps=10, warmup=2):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    out = []
+    for _ in range(steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        fn()
+        b.record(s)
+        b.synchronize()
+        out.append(a.elapsed_time(b))
+    return statistics.median(out)
+
+
+def tune(order, bases, G, u, v, steps=10):
+    """Per order, the elements-per-workgroup divisor (1 or 2) whose launch of
+    that order's tiles is fastest; returns (div_by_order, {p: {div: ms}})."""
+    plans = {d: MixedPlan(order, div_by_order=[d] * 9) for d in (1, 2)}
+    best = [1] * 9
+    times = {}
+    for p in range(1, 9):
+        if not np.any(order == p):
+            continue
+        times[p] = {d: _median_ms(lambda: pl.apply_order(p, bases, G, u, v), steps) for d, pl in plans.items()}
+        best[p] = min(times[p], key=times[p].get)
+    for pl in plans.values():
+        pl.close()
+    return best, times

@@ -1,0 +1,117 @@
+"""Mixed-order BK3 batches (p = 1..8 in one E-vector): the single launch,
+the persistent launch, the per-order launches and their hipGraphs, against
+the FP64 oracle per order (<= 1e-13) and against each other."""
+import numpy as np
+import pytest
+import torch
+
+import _oracle as O
+from omp_bk_amd import geometry as geo
+from omp_bk_amd import mixed
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda:0"
+
+
+def _batch(nelmt, seed):
+    order = mixed.zipf_orders(nelmt, seed=seed)
+    plan = mixed.MixedPlan(order)
+    bases = mixed.packed_bases()
+    G = mixed.affine_factors(order, plan.goff, plan.ngeo, seed=seed)
+    u = np.random.default_rng(seed).uniform(-1, 1, plan.ndof)
+    return order, plan, bases, G, u
+
+
+def _oracle_check(order, plan, bases, G, u, v, elems=None):
+    """Each order's elements (or the given subset) against bko_bk3_sumfact."""
+    boff = 0
+    worst = 0.0
+    for p in range(1, 9):
+        n, q = p + 1, p + 2
+        B = bases[boff:boff + n * q].reshape(n, q)
+        D = bases[boff + n * q:boff + n * q + q * q].reshape(q, q)
+        boff += n * q + q * q
+        idx = np.nonzero(order == p)[0]
+        if elems is not None:
+            idx = np.intersect1d(idx, elems)
+        if idx.size == 0:
+            continue
+        ui = np.concatenate([u[plan.uoff[e]:plan.uoff[e] + n ** 3] for e in idx])
+        Gi = np.concatenate([G[plan.goff[e]:plan.goff[e] + 6 * q ** 3] for e in idx])
+        vi = np.concatenate([v[plan.uoff[e]:plan.uoff[e] + n ** 3] for e in idx])
+        ref = O.bk3(q, B, D, Gi, ui)
+        assert np.all(np.isfinite(vi))
+        err = O.rel_maxdiff(vi, ref)
+        worst = max(worst, err)
+        assert err <= 1e-13, (p, err)
+    return worst
+
+
+def _t(x):
+    return torch.as_tensor(np.ascontiguousarray(x), device=DEV)
+
+
+@pytest.mark.parametrize("nelmt", [1, 9, 517])
+def test_small_batches(nelmt):
+    order, plan, bases, G, u = _batch(nelmt, seed=nelmt)
+    b, g, uu = _t(bases), _t(G), _t(u)
+    v1 = torch.full_like(uu, float("nan"))
+    plan.apply(b, g, uu, v1)
+    v2 = torch.full_like(uu, float("nan"))
+    plan.apply_by_order(b, g, uu, v2)
+    torch.cuda.synchronize()
+    _oracle_check(order, plan, bases, G, u, v1.cpu().numpy())
+    assert torch.equal(v1, v2)
+    persist = mixed.MixedPlan(order, rounds_by_order=[2] * 9, grid=64)
+    v3 = torch.full_like(uu, float("nan"))
+    persist.apply(b, g, uu, v3)
+    torch.cuda.synchronize()
+    assert torch.equal(v1, v3)
+
+
+def test_tuned_divisors():
+    order, plan, bases, G, u = _batch(3000, seed=5)
+    b, g, uu = _t(bases), _t(G), _t(u)
+    v = torch.empty_like(uu)
+    div, times = mixed.tune(order, b, g, uu, v, steps=3)
+    tuned = mixed.MixedPlan(order, div_by_order=div)
+    vt = torch.full_like(uu, float("nan"))
+    tuned.apply(b, g, uu, vt)
+    torch.cuda.synchronize()
+    _oracle_check(order, tuned, bases, G, u, vt.cpu().numpy())
+
+
+def test_graph_replays():
+    """Each replay of the captured launch, with eager work between replays,
+    equals the eager result."""
+    order, plan, bases, G, u = _batch(4000, seed=3)
+    b, g, uu = _t(bases), _t(G), _t(u)
+    ref = torch.empty_like(uu)
+    plan.apply(b, g, uu, ref)
+    for by_order in (False, True):
+        v = torch.full_like(uu, float("nan"))
+        graph = plan.graph(b, g, uu, v, by_order=by_order)
+        for rep in range(4):
+            v.fill_(float("nan"))
+            graph.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(v, ref), (by_order, rep)
+            _ = (uu * 2.0).sum().item()      # eager work between replays
+        graph.close()
+
+
+def test_full_batch_sampled():
+    """The 200,000-element Zipf batch: sampled elements of every order
+    against the oracle; single launch == per-order launches."""
+    order, plan, bases, G, u = _batch(200000, seed=2024)
+    b, g, uu = _t(bases), _t(G), _t(u)
+    v1 = torch.full_like(uu, float("nan"))
+    v2 = torch.full_like(uu, float("nan"))
+    plan.apply(b, g, uu, v1)
+    plan.apply_by_order(b, g, uu, v2)
+    torch.cuda.synchronize()
+    assert torch.equal(v1, v2)
+    assert torch.isfinite(v1).all()
+    rng = np.random.default_rng(0)
+    sample = np.concatenate([rng.choice(np.nonzero(order == p)[0], 40, replace=False) for p in range(1, 9)])
+    _oracle_check(order, plan, bases, G, u, v1.cpu().numpy(), elems=sample)

@@ -33,6 +33,7 @@ def test_variant(op, form, variant, p):
     L = _lib.lib()
     f = FORM[form]
     E = 13
+    prev = L.bkx_tune_get(op, f, p)
     _lib.call("bkx_tune_set", op, f, p, variant)
     try:
         if op == 5:
@@ -62,5 +63,5 @@ def test_variant(op, form, variant, p):
         assert np.all(np.isfinite(got))
         assert O.rel_maxdiff(got, ref) <= 1e-13
     finally:
-        _lib.call("bkx_tune_set", op, f, p, 0)
-        assert L.bkx_tune_get(op, f, p) == 0
+        _lib.call("bkx_tune_set", op, f, p, prev)
+        assert L.bkx_tune_get(op, f, p) == prev
