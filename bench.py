@@ -29,6 +29,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk3", "copy"], default=None,
+                    help="run one workload only (for profiling) and print its timing")
     return ap.parse_args()
 
 
@@ -82,6 +84,37 @@ def bk3_workload(p=7, E=16384):
     v = torch.empty_like(u)
     fn = lambda: ops.bk3(p, B, D, G, u, out=v)
     return dict(fn=fn, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + 6 * E * q ** 3), p=p, E=E)
+
+
+def mixed_batch(steps, warmup, nelmt=200000):
+    """BK3 over 200,000 elements of mixed order (Zipf(1.2) over p = 1..8):
+    the per-order tuned single launch and the per-order launches, both
+    replayed from hipGraphs."""
+    import numpy as np
+    import torch
+    from omp_bk_amd import mixed
+    dev = "cuda:0"
+    order = mixed.zipf_orders(nelmt)
+    plan0 = mixed.MixedPlan(order)
+    bases = torch.as_tensor(mixed.packed_bases(), device=dev)
+    G = mixed.affine_factors(order, plan0.goff, plan0.ngeo, device=dev)
+    torch.manual_seed(99)
+    u = torch.rand(plan0.ndof, dtype=torch.float64, device=dev) * 2 - 1
+    v = torch.empty_like(u)
+    div, _ = mixed.tune(order, bases, G, u, v, steps=5)
+    plan = mixed.MixedPlan(order, div_by_order=div)
+    nbytes = 8 * (2 * plan.ndof + plan.ngeo)
+    out = {"nelmt": nelmt, "counts_by_p": np.bincount(order, minlength=9)[1:].tolist(),
+           "div_by_p": div[1:], "ntiles": plan.ntiles, "bytes": nbytes}
+    for name, by_order in (("single_launch_graph", False), ("per_order_graph", True)):
+        g = plan.graph(bases, G, u, v, by_order=by_order)
+        ms, _ = time_steps(g.launch, steps, warmup)
+        g.close()
+        out[name] = {"ms": round(ms, 4), "GDoF/s": round(plan.ndof / (ms * 1e-3) * 1e-9, 3),
+                     "GB/s": round(nbytes / (ms * 1e-3) * 1e-9, 1)}
+    plan.close()
+    plan0.close()
+    return out
 
 
 def reduced_forms(w, steps, warmup, ms64):
@@ -180,6 +213,34 @@ def traffic(name):
         return None
 
 
+def run_only(a):
+    """One workload, a few steps, for rocprofv3 runs."""
+    from omp_bk_amd import _lib
+    from omp_bk_amd import ops
+    import torch
+    if a.only == "copy":
+        n = (2 << 30) // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        fn = lambda: _lib.call("bkx_probe_copy", x.data_ptr(), y.data_ptr(), n,
+                               torch.cuda.current_stream().cuda_stream)
+        nbytes = 2 * 8 * n
+    elif a.only == "bk3":
+        w = bk3_workload()
+        fn, nbytes = w["fn"], w["bytes"]
+    else:
+        w = bk5_workload()
+        fn, nbytes = w["fn"], w["bytes"]
+        if a.only != "bk5":
+            form = a.only.split("_")[1]
+            G32, u32 = w["G"].float(), w["u"].float()
+            fn = lambda: ops.bk5(w["p"], w["D"], G32, u32, out=w["v"], form=form)
+            nbytes = 36 * w["dofs"]
+    ms, _ = time_steps(fn, a.steps, a.warmup)
+    print(json.dumps({"only": a.only, "ms_per_step": round(ms, 5), "GB/s": round(nbytes / (ms * 1e-3) * 1e-9, 1),
+                      "bytes_per_step": nbytes}), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -189,6 +250,10 @@ def main():
     import torch
     torch.cuda.init()
 
+    if a.only:
+        run_only(a)
+        return
+
     roofs = probe_roofs()
     w5 = bk5_workload()
     ms5, wall5 = time_steps(w5["fn"], a.steps, a.warmup)
@@ -196,6 +261,8 @@ def main():
     gbs5 = w5["bytes"] / (ms5 * 1e-3) * 1e-9
 
     forms = reduced_forms(w5, a.steps, a.warmup, ms5)
+
+    mix = mixed_batch(a.steps, a.warmup)
 
     w3 = bk3_workload()
     ms3, _ = time_steps(w3["fn"], a.steps, a.warmup)
@@ -227,6 +294,7 @@ def main():
         "frac_of_read7": round(gbs5 / roofs["read7"], 4),
         "wall_ms_per_step": round(wall5, 5),
         **forms,
+        "bk3_mixed": mix,
         "bk3_p7_f64": {"value": round(gdofs3, 3), "unit": "GDoF/s", "nelmt": 16384, "ms_per_step": round(ms3, 5),
                        "achieved_GBs": round(gbs3, 1), "frac": round(gbs3 / HBM_PEAK_GBS, 4),
                        "bytes_per_dof": round(8 * (2 + 6 * (9 / 8) ** 3), 3)},

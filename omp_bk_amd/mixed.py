@@ -86,7 +86,9 @@ def packed_bases():
 def affine_factors(order, goff, ngeo, amp=0.1, seed=7, device=None):
     """G of every element of a mixed batch: element e is x = x0 + A_e xi with
     A_e = h (I + amp R_e), h = 1 / cbrt(nelmt); G = w detJ J^-1 J^-T (J = A_e,
-    constant over the element) at its Gauss points, factor-major."""
+    constant over the element) at its Gauss points, factor-major.  With
+    ``device`` the factors are scattered into a device tensor there (the
+    per-element matrices are made on the host either way)."""
     nelmt = order.size
     rng = np.random.default_rng(seed)
     h = 1.0 / np.cbrt(nelmt)
@@ -94,7 +96,13 @@ def affine_factors(order, goff, ngeo, amp=0.1, seed=7, device=None):
     inv = np.linalg.inv(A)
     M = np.einsum("eab,ecb->eac", inv, inv) * np.linalg.det(A)[:, None, None]
     sym = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
-    G = np.empty(ngeo)
+    M6 = np.stack([M[:, a, b] for a, b in sym], axis=1)      # (nelmt, 6)
+    if device is None:
+        G = np.empty(ngeo)
+    else:
+        G = torch.empty(ngeo, dtype=torch.float64, device=device)
+        M6_d = torch.as_tensor(M6, device=device)
+        goff_d = torch.as_tensor(goff, device=device)
     for p in range(1, 9):
         idx = np.nonzero(order == p)[0]
         if idx.size == 0:
@@ -102,10 +110,16 @@ def affine_factors(order, goff, ngeo, amp=0.1, seed=7, device=None):
         q = p + 2
         _, w = geo.gauss(q)
         w3 = (w[:, None, None] * w[None, :, None] * w[None, None, :]).reshape(-1)
-        vals = np.stack([M[idx, a, b] for a, b in sym], axis=1)[:, :, None] * w3[None, None, :]   # (k, 6, q^3)
-        pos = goff[idx][:, None] + np.arange(6 * q ** 3)[None, :]
-        G[pos] = vals.reshape(idx.size, -1)
-    return G if device is None else torch.as_tensor(G, device=device)
+        if device is None:
+            vals = M6[idx][:, :, None] * w3[None, None, :]              # (k, 6, q^3)
+            pos = goff[idx][:, None] + np.arange(6 * q ** 3)[None, :]
+            G[pos] = vals.reshape(idx.size, -1)
+        else:
+            ii = torch.as_tensor(idx, device=device)
+            vals = M6_d[ii][:, :, None] * torch.as_tensor(w3, device=device)[None, None, :]
+            pos = goff_d[ii][:, None] + torch.arange(6 * q ** 3, device=device)[None, :]
+            G[pos.reshape(-1)] = vals.reshape(-1)
+    return G
 
 
 # ---------------------------------------------------------------------------

@@ -220,6 +220,29 @@ def test_bk1_ff_accuracy(p):
     assert err < 1e-13
 
 
+def test_kappa_operator_and_metrics():
+    """kappa = 1 is BK3 bit for bit; the operator is linear in kappa; the
+    norms and error metrics against numpy's closed forms."""
+    p, E = 4, 5
+    nm, nq = p + 1, p + 2
+    B, D = geo.bk13_bases(p)
+    G = geo.geometric_factors(3, p, 8)[:E]
+    rng = np.random.default_rng(1)
+    u = rng.uniform(-1, 1, E * nm ** 3)
+    k1, k2 = np.exp(rng.standard_normal((2, E * nq ** 3)))
+    assert np.array_equal(O.bk3_kappa(nq, B, D, G, np.ones(E * nq ** 3), u), O.bk3(nq, B, D, G, u))
+    lin = O.bk3_kappa(nq, B, D, G, 2.0 * k1 + 3.0 * k2, u)
+    ref = 2.0 * O.bk3_kappa(nq, B, D, G, k1, u) + 3.0 * O.bk3_kappa(nq, B, D, G, k2, u)
+    assert np.abs(lin - ref).max() <= 1e-13 * np.abs(ref).max()
+    x, r = rng.standard_normal((2, 1000))
+    import ctypes
+    dp = ctypes.POINTER(ctypes.c_double)
+    n2 = O.oracle().bko_norm2(x.ctypes.data_as(dp), ctypes.c_size_t(x.size))
+    assert abs(n2 - np.linalg.norm(x)) <= 1e-14 * n2
+    assert abs(O.rel_l2(x, r) - np.linalg.norm(x - r) / np.linalg.norm(r)) <= 1e-14
+    assert O.rel_maxdiff(x, r) == np.abs(x - r).max() / np.abs(r).max()
+
+
 # ---- adjoint -------------------------------------------------------------------
 
 @pytest.mark.parametrize("p", [2, 5])
