@@ -29,7 +29,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk3", "copy"], default=None,
+    ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk5_bf16s", "bk3", "copy"], default=None,
                     help="run one workload only (for profiling) and print its timing")
     return ap.parse_args()
 
@@ -128,17 +128,18 @@ def reduced_forms(w, steps, warmup, ms64):
     G32, u32 = w["G"].float(), w["u"].float()
     ref_q = ops.bk5(p, w["D"], G32.double(), u32.double())
     out = {}
-    for form in ("f32c", "f32w"):
-        v = torch.empty_like(w["v"])
+    for form in ("f32c", "f32w", "bf16s"):
+        v = torch.empty_like(w["v"], dtype=torch.float32 if form == "bf16s" else torch.float64)
+        nb = 32 if form == "bf16s" else 36
         fn = lambda: ops.bk5(p, w["D"], G32, u32, out=v, form=form)
         ms, _ = time_steps(fn, steps, warmup)
         big = ref_q.abs().max()
         out["bk5_p7_" + form] = {
             "value": round(w["dofs"] / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
-            "speedup_vs_f64": round(ms64 / ms, 3), "bytes_per_dof": 36,
-            "frac": round(36 * w["dofs"] / (ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
-            "relerr_vs_f64_same_inputs": float(((v - ref_q).abs().max() / big).item()),
-            "relerr_vs_f64_unrounded_inputs": float(((v - w["v"]).abs().max() / w["v"].abs().max()).item()),
+            "speedup_vs_f64": round(ms64 / ms, 3), "bytes_per_dof": nb,
+            "frac": round(nb * w["dofs"] / (ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
+            "relerr_vs_f64_same_inputs": float(((v.double() - ref_q).abs().max() / big).item()),
+            "relerr_vs_f64_unrounded_inputs": float(((v.double() - w["v"]).abs().max() / w["v"].abs().max()).item()),
         }
         del v
     del G32, u32, ref_q
@@ -234,8 +235,9 @@ def run_only(a):
         if a.only != "bk5":
             form = a.only.split("_")[1]
             G32, u32 = w["G"].float(), w["u"].float()
-            fn = lambda: ops.bk5(w["p"], w["D"], G32, u32, out=w["v"], form=form)
-            nbytes = 36 * w["dofs"]
+            vo = w["v"] if form != "bf16s" else torch.empty_like(u32)
+            fn = lambda: ops.bk5(w["p"], w["D"], G32, u32, out=vo, form=form)
+            nbytes = (32 if form == "bf16s" else 36) * w["dofs"]
     ms, _ = time_steps(fn, a.steps, a.warmup)
     print(json.dumps({"only": a.only, "ms_per_step": round(ms, 5), "GB/s": round(nbytes / (ms * 1e-3) * 1e-9, 1),
                       "bytes_per_step": nbytes}), flush=True)

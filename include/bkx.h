@@ -115,6 +115,14 @@ int bkx_bk5_apply_f32c(int p, size_t nelmt, const double* dbasis, const float* G
 int bkx_bk5_apply_f32w(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, double* out,
                        bkx_stream_t stream);
 
+/* Split-BF16 MFMA form, p = 7 only (n = 8 fills the 16 x 16 x 16 MFMA tiles):
+ * every contraction operand is split into two BF16 parts (about 16
+ * significant bits) and the products are accumulated in FP32 by
+ * v_mfma_f32_16x16x16_bf16; fluxes in FP32.  u, G and v are FP32 (32 bytes
+ * per DoF).  A reduced-precision form: about 1e-5 relative error. */
+int bkx_bk5_apply_bf16s(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, float* out,
+                        bkx_stream_t stream);
+
 /* ---------------------------------------------------------------------- */
 /* BK3 with a coefficient and its adjoint (no reference counterpart; the    */
 /* operator is that of bkx_bk3_apply_f64 with each point's six factors      */

@@ -23,6 +23,7 @@ int bk3_adjoint_f64(int p, size_t nelmt, const double* B, const double* D, const
 int bk3_vjp_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
                 const double* u, const double* w, double* gu, double* gk, hipStream_t s);
 int sum_f64(const double* x, size_t n, double* out, hipStream_t s);
+int bk5_apply_bf16s(int p, size_t nelmt, const double* D, const float* G, const float* u, float* v, hipStream_t s);
 int bk5_variants();
 int bk5_apply_f64(int variant, int p, size_t nelmt, const double* D, const double* G, const double* u, double* v,
                   hipStream_t s);
@@ -229,6 +230,21 @@ int bkx_bk5_apply_f32w(int p, size_t nelmt, const double* dbasis, const float* G
         return BKX_ERROR_NULL;
     }
     return bkx::bk5_apply_f32w(variant_of(5, BKX_FORM_F32W, p), p, nelmt, dbasis, G, in, out, as_stream(stream));
+}
+
+int bkx_bk5_apply_bf16s(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, float* out,
+                        bkx_stream_t stream)
+{
+    if (p != 7) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!dbasis || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk5_apply_bf16s(p, nelmt, dbasis, G, in, out, as_stream(stream));
 }
 
 int bkx_bk3_apply_kappa_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,

@@ -87,6 +87,7 @@ _BK5_FORMS = {
     "f64": ("bkx_bk5_apply_f64", torch.float64),
     "f32c": ("bkx_bk5_apply_f32c", torch.float32),
     "f32w": ("bkx_bk5_apply_f32w", torch.float32),
+    "bf16s": ("bkx_bk5_apply_bf16s", torch.float32),
 }
 
 
@@ -95,7 +96,9 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
 
     form "f64": u, G, v in FP64 (the drop-in);
          "f32c": u, G in FP32, float-float compensated arithmetic, v FP64;
-         "f32w": u, G in FP32, FP64 arithmetic, v FP64."""
+         "f32w": u, G in FP32, FP64 arithmetic, v FP64;
+         "bf16s": u, G in FP32, split-BF16 MFMA contractions accumulated in
+                  FP32 (p = 7 only), v FP32."""
     _order_ok(5, p)
     if form not in _BK5_FORMS:
         raise ValueError("unknown BK5 form %r (one of %s)" % (form, ", ".join(_BK5_FORMS)))
@@ -107,9 +110,10 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
     _check("dbasis", dbasis, nq * nq)
     _check("G", G, ne * 6 * nq ** 3, sdt)
     _check("u", u, ne * nq ** 3, sdt)
+    odt = torch.float32 if form == "bf16s" else torch.float64
     if out is None:
-        out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
-    _check("out", out, ne * nq ** 3)
+        out = torch.empty(u.shape, dtype=odt, device=u.device)
+    _check("out", out, ne * nq ** 3, odt)
     with torch.cuda.device(u.device):
         _lib.call(entry, p, ne, dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
     return out

@@ -87,3 +87,28 @@ def test_full_size_sampled(form):
     # against unquantised inputs the error is the FP32 rounding of u and G
     e64 = ((v - v64).abs().max() / v64.abs().max()).item()
     assert 1e-10 < e64 < 1e-5
+
+
+def test_bf16s_error_and_layout():
+    """The split-BF16 MFMA form at p = 7 against the FP64 oracle on the same
+    FP32 inputs: a reduced-precision form, bounded at 1e-4 max-relative
+    (16-bit operands, FP32 sums); a wrong MFMA operand layout or a missed
+    direction gives O(1)."""
+    p, E = 7, 37
+    D, G, u = _case(p, E, seed=4)
+    Gq = G.astype(np.float32).astype(np.float64)
+    uq = u.astype(np.float32).astype(np.float64)
+    ref = O.bk5(p + 1, D, Gq, uq)
+    out = torch.full((u.size,), float("nan"), dtype=torch.float32, device=DEV)
+    ops.bk5(p, torch.as_tensor(D, device=DEV), torch.as_tensor(G.astype(np.float32), device=DEV),
+            torch.as_tensor(u.astype(np.float32), device=DEV), out=out, form="bf16s")
+    got = out.double().cpu().numpy()
+    assert np.all(np.isfinite(got))
+    err = O.rel_maxdiff(got, ref)
+    assert 1e-9 < err < 1e-4, err
+
+
+def test_bf16s_rejects_other_orders():
+    with pytest.raises(Exception):
+        ops.bk5(6, torch.zeros(49, dtype=torch.float64, device=DEV), torch.zeros(6 * 343, device=DEV),
+                torch.zeros(343, device=DEV), form="bf16s")
