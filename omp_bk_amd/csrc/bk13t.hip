@@ -38,25 +38,26 @@ namespace {
 using bk13t::Bk13tShape;
 using bk13t::bk13t_body;
 
-template <int NM, int NQ, bool BK3, int DIV, bool SB>
-__global__ __launch_bounds__((Bk13tShape<NM, NQ, DIV>::threads)) void bk13t_kernel(
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF>
+__global__ __launch_bounds__((Bk13tShape<NM, NQ, DIV>::threads))
+__attribute__((amdgpu_waves_per_eu(GPF > 0 ? 4 : 1))) void bk13t_kernel(
     size_t nelmt, const double* __restrict__ B, const double* __restrict__ D, const double* __restrict__ Q,
     const double* __restrict__ u, double* __restrict__ v)
 {
     using S = Bk13tShape<NM, NQ, DIV>;
-    __shared__ double s_box[S::epb * 3 * S::box];
+    __shared__ double s_box[S::epb * NB * S::box];
     const int slot = threadIdx.x / S::nq2;
     const int l = threadIdx.x - slot * S::nq2;
     const size_t e = size_t(blockIdx.x) * S::epb + slot;
     const bool active = slot < S::epb && e < nelmt;
     const size_t ee = active ? e : 0;
     constexpr size_t qper = BK3 ? 6 * S::nq3 : S::nq3;
-    bk13t_body<NM, NQ, BK3, DIV, SB>(B, D, Q + ee * qper, u + ee * S::nm3, v + ee * S::nm3, active, slot, l, s_box);
+    bk13t_body<NM, NQ, BK3, DIV, SB, NB, GPF>(B, D, Q + ee * qper, u + ee * S::nm3, v + ee * S::nm3, active, slot, l, s_box);
 }
 
 }  // namespace
 
-template <int NM, int NQ, bool BK3, int DIV, bool SB>
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF = 0>
 int launch_bk13t(size_t nelmt, const double* B, const double* D, const double* Q, const double* u, double* v,
                  hipStream_t s)
 {
@@ -65,7 +66,7 @@ int launch_bk13t(size_t nelmt, const double* B, const double* D, const double* Q
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk13t_kernel<NM, NQ, BK3, DIV, SB>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, Q, u, v);
+    hipLaunchKernelGGL((bk13t_kernel<NM, NQ, BK3, DIV, SB, NB, GPF>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, Q, u, v);
     return launch_status();
 }
 
@@ -73,13 +74,28 @@ template <int NM, bool BK3>
 int launch_bk13t_variant(int div_variant, size_t nelmt, const double* B, const double* D, const double* Q,
                          const double* u, double* v, hipStream_t s)
 {
+    // variant: 0 all elements per workgroup, 1 half of them, 2 all with a
+    // scheduling barrier per row (three LDS boxes each); 4 all, 5 half (two
+    // LDS boxes); BK3 only: 6 all, 7 half (two boxes, three rows of G loaded
+    // with u), 8 all (two boxes, two rows); BK1 6, 7, 8 are 4, 5, 4.
+    // Variant 3 is the LDS-contraction mapping of bk13_f64.hip.
     switch (div_variant) {
     case 0:
-        return launch_bk13t<NM, NM + 1, BK3, 1, false>(nelmt, B, D, Q, u, v, s);
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 3>(nelmt, B, D, Q, u, v, s);
     case 1:
-        return launch_bk13t<NM, NM + 1, BK3, 2, false>(nelmt, B, D, Q, u, v, s);
+        return launch_bk13t<NM, NM + 1, BK3, 2, false, 3>(nelmt, B, D, Q, u, v, s);
     case 2:
-        return launch_bk13t<NM, NM + 1, BK3, 1, true>(nelmt, B, D, Q, u, v, s);
+        return launch_bk13t<NM, NM + 1, BK3, 1, true, 3>(nelmt, B, D, Q, u, v, s);
+    case 4:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2>(nelmt, B, D, Q, u, v, s);
+    case 5:
+        return launch_bk13t<NM, NM + 1, BK3, 2, false, 2>(nelmt, B, D, Q, u, v, s);
+    case 6:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, BK3 ? 3 : 0>(nelmt, B, D, Q, u, v, s);
+    case 7:
+        return launch_bk13t<NM, NM + 1, BK3, 2, false, 2, BK3 ? 3 : 0>(nelmt, B, D, Q, u, v, s);
+    case 8:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, BK3 ? 2 : 0>(nelmt, B, D, Q, u, v, s);
     default:
         return BKX_ERROR_ARG;
     }

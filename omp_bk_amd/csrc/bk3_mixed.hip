@@ -40,12 +40,16 @@ struct MixedTile {
 };
 
 constexpr int kMixedThreads = 256;
+// every order's body: two LDS boxes per element, two rows of G loaded ahead
+// (the fastest single-order BK3 form at most orders)
+constexpr int kMixedBoxes = 2;
+constexpr int kMixedGpf = 2;
 
 template <int P>
 constexpr int smem_of()
 {
     using S = bk13t::Bk13tShape<P + 1, P + 2, 1>;
-    return S::epb * 3 * S::box;
+    return S::epb * kMixedBoxes * S::box;
 }
 
 template <int P>
@@ -78,15 +82,15 @@ __device__ __forceinline__ void run_tile(int tid, const MixedTile& t, const int3
         // one round: the tile holds at most epb elements
         const bool active = slot < S::epb && slot < t.count;
         const int e = elems[t.first + (active ? slot : 0)];
-        bk13t::bk13t_body<NM, NQ, true, DIV, false>(B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l,
-                                                    smem);
+        bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf>(B, D, G + goff[e], u + uoff[e],
+                                                                            v + uoff[e], active, slot, l, smem);
     } else {
         for (int r0 = 0; r0 < t.count; r0 += S::epb) {
             const int idx = r0 + slot;
             const bool active = slot < S::epb && idx < t.count;
             const int e = elems[t.first + (active ? idx : 0)];
-            bk13t::bk13t_body<NM, NQ, true, DIV, false>(B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l,
-                                                        smem, t.first + r0);
+            bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf>(
+                B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l, smem, t.first + r0);
             __syncthreads();
         }
     }

@@ -41,7 +41,7 @@ hipStream_t as_stream(bkx_stream_t s) { return reinterpret_cast<hipStream_t>(s);
 
 // Launch variant of each (operator, form, order).  The defaults are the
 // fastest variants measured on an MI355X by omp_bk_amd/tune.py (BK5 over
-// E = 65536, BK3 over 16384, BK1 over 32768 elements); the autotuner
+// E = 65536, BK1 and BK3 over 32768 elements); the autotuner
 // overwrites them at run time.
 constexpr int kOps = 6;
 constexpr int kForms = 4;
@@ -61,8 +61,8 @@ struct VariantTable {
         //                            p:   0   1   2   3   4   5   6   7   8
         static const int bk5_f64[kOrders] = {0, 20, 1, 21, 15, 17, 20, 4, 0};
         static const int bk5_f32w[kOrders] = {0, 17, 17, 17, 17, 17, 17, 17, 0};
-        static const int bk3_f64[kOrders] = {0, 1, 1, 1, 1, 1, 1, 1, 1};
-        static const int bk1_f64[kOrders] = {0, 3, 3, 3, 3, 1, 1, 1, 1};
+        static const int bk3_f64[kOrders] = {0, 1, 8, 6, 7, 6, 8, 6, 7};
+        static const int bk1_f64[kOrders] = {0, 1, 5, 5, 5, 4, 5, 4, 5};
         for (int p = 0; p < kOrders; ++p) {
             v[5][BKX_FORM_F64][p].store(bk5_f64[p], std::memory_order_relaxed);
             v[5][BKX_FORM_F32W][p].store(bk5_f32w[p], std::memory_order_relaxed);
@@ -115,9 +115,12 @@ int bkx_tune_count(int op, int form)
         return bkx::bk5_variants();
     }
     if ((op == 1 || op == 3) && form == BKX_FORM_F64) {
-        // 0-2: transposing mapping (bk13t.hip) with all, half or a third of
-        // the elements per workgroup; 3: LDS-contraction mapping (bk13_f64.hip)
-        return 4;
+        // transposing mapping (bk13t.hip): 0 all elements per workgroup,
+        // 1 half, 2 all with a scheduling barrier per row (three LDS boxes
+        // per element), 4 all, 5 half (two LDS boxes), 6 all, 7 half, 8 all
+        // (two boxes, BK3 with three, three and two rows of G loaded ahead;
+        // BK1: as 4, 5, 4); 3: LDS-contraction mapping (bk13_f64.hip)
+        return 9;
     }
     return (op == 1 || op == 3 || op == 5) && form >= 0 && form < kForms ? 1 : -1;
 }

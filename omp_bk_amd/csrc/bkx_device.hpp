@@ -42,10 +42,13 @@ __device__ __forceinline__ void pin(float& x) { asm volatile("" : "+v"(x)); }
 // loads); TAG makes each phase's asm text distinct so that two launders of
 // the same pointer are not merged.  An inline-asm result counts as
 // divergent, so the pointer is made provably uniform again with
-// readfirstlane, and it is returned in the global address space (a generic
-// pointer would be read with vector flat loads).
+// readfirstlane, and it is returned in the constant address space: the
+// matrices are never written by a kernel, and loads from that space stay
+// scalar loads even after stores in a loop (through a global pointer they
+// would need a proof that no earlier store clobbers them, and through a
+// generic pointer they would be vector flat loads).
 template <typename T>
-using global_ptr = const T __attribute__((address_space(1)))*;
+using global_ptr = const T __attribute__((address_space(4)))*;
 
 // dep (a wavefront-uniform value) makes the launder depend on it, so that
 // in a loop the loads are not hoisted out when dep is the loop counter.
