@@ -51,7 +51,9 @@ enum bkx_form {
     BKX_FORM_F64 = 0,    /* FP64 storage and arithmetic (the drop-in) */
     BKX_FORM_F32C = 1,   /* FP32 storage, float-float compensated arithmetic */
     BKX_FORM_F32W = 2,   /* FP32 storage, FP64 arithmetic */
-    BKX_FORM_BF16S = 3   /* FP32 storage, split-BF16 MFMA products, FP32 accumulation */
+    BKX_FORM_BF16S = 3,  /* FP32 storage, split-BF16 MFMA products, FP32 accumulation */
+    BKX_FORM_FF = 4      /* FP64 storage split into float-float, float-float arithmetic
+                            (the reference's BK1_ff kernels, openmp/BK1_ff.cpp:73-723) */
 };
 
 /* Human-readable name of a bkx_status. */
@@ -122,6 +124,37 @@ int bkx_bk5_apply_f32w(int p, size_t nelmt, const double* dbasis, const float* G
  * per DoF).  A reduced-precision form: about 1e-5 relative error. */
 int bkx_bk5_apply_bf16s(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, float* out,
                         bkx_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
+/* Float-float and reduced-storage forms of BK1 and BK3 (v is FP64)         */
+/* ---------------------------------------------------------------------- */
+
+/* Float-float arithmetic on FP64 storage: u and JxW are split into float-
+ * float (hi + lo) when loaded, every contraction is a Dot2 compensated sum,
+ * v is joined to FP64.  Replaces the reference's float-float BK1 kernels
+ * SumFactorizationFF / SumFactorizationFFTeams / SumFactorizationFFPlane
+ * (openmp/BK1_ff.cpp:73-241, 265-445, 501-723), which take the FP64
+ * interface of bk::SumFactorization<double, nq>.  Error vs the FP64
+ * operator ~1e-15 relative. */
+int bkx_bk1_apply_ff(int p, size_t nelmt, const double* basis, const double* JxW, const double* in, double* out,
+                     bkx_stream_t stream);
+
+/* BK1 with u and JxW stored in FP32: float-float arithmetic (f32c) or FP64
+ * arithmetic (f32w); no reference counterpart. */
+int bkx_bk1_apply_f32c(int p, size_t nelmt, const double* basis, const float* JxW, const float* in, double* out,
+                       bkx_stream_t stream);
+int bkx_bk1_apply_f32w(int p, size_t nelmt, const double* basis, const float* JxW, const float* in, double* out,
+                       bkx_stream_t stream);
+
+/* BK3 in the same three forms: float-float on FP64 storage (the BK1_ff
+ * scheme applied to openmp/bk3_sumfact.h:17-187), and with u and G stored
+ * in FP32 with float-float (f32c) or FP64 (f32w) arithmetic. */
+int bkx_bk3_apply_ff(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                     const double* in, double* out, bkx_stream_t stream);
+int bkx_bk3_apply_f32c(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                       const float* in, double* out, bkx_stream_t stream);
+int bkx_bk3_apply_f32w(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                       const float* in, double* out, bkx_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* BK3 with a coefficient and its adjoint (no reference counterpart; the    */

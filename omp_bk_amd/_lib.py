@@ -21,6 +21,12 @@ _SIGNATURES = {
     "bkx_bk5_apply_f32c": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk5_apply_f32w": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk5_apply_bf16s": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk1_apply_ff": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk1_apply_f32c": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk1_apply_f32w": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk3_apply_ff": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk3_apply_f32c": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk3_apply_f32w": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk3_apply_kappa_f64": [_c_int, _c_size] + [_vp] * 7,
     "bkx_bk3_adjoint_f64": [_c_int, _c_size] + [_vp] * 10,
     "bkx_bk3_vjp_f64": [_c_int, _c_size] + [_vp] * 9,

@@ -4,7 +4,9 @@
 // The mapping is described in bk13t.hip.
 #pragma once
 
+#include "bkx.h"
 #include "bkx_device.hpp"
+#include "bkx_ff.hpp"
 
 namespace bkx {
 namespace bk13t {
@@ -25,6 +27,108 @@ struct Bk13tShape {
     static constexpr int box = NQ * NQ * row;
 };
 
+// ---- arithmetic forms ------------------------------------------------------
+// FORM (enum bkx_form): storage type TS of u and JxW/G, value type V of the
+// registers and LDS boxes, and the type L a loaded value is kept in:
+//   F64   TS double, V double, L double   (the reference's SumFactorization<double>)
+//   F32W  TS float,  V double, L float    (FP32 storage, FP64 arithmetic)
+//   F32C  TS float,  V ff,     L float    (FP32 storage, float-float arithmetic)
+//   FF    TS double, V ff,     L ff       (FP64 storage split into float-float
+//                                          when loaded, float-float arithmetic:
+//                                          the reference's BK1_ff kernels)
+// v is stored in FP64 by every form (a float-float value joined with one add).
+// B and D are FP64 for the double forms (wavefront-uniform scalar loads) and
+// float-float for the others (a per-workgroup LDS copy, split once).
+template <int FORM>
+struct Form;
+template <>
+struct Form<BKX_FORM_F64> {
+    using TS = double;
+    using V = double;
+    using L = double;
+};
+template <>
+struct Form<BKX_FORM_F32W> {
+    using TS = float;
+    using V = double;
+    using L = float;
+};
+template <>
+struct Form<BKX_FORM_F32C> {
+    using TS = float;
+    using V = ff;
+    using L = float;
+};
+template <>
+struct Form<BKX_FORM_FF> {
+    using TS = double;
+    using V = ff;
+    using L = ff;
+};
+
+__device__ __forceinline__ double load_as(const double* p, double*) { return *p; }
+__device__ __forceinline__ float load_as(const float* p, float*) { return *p; }
+__device__ __forceinline__ ff load_as(const double* p, ff*) { return ff_split(*p); }
+
+// Sum of products, in ascending order of the terms.  FP64: a running FMA
+// chain (the first product rounded alone: -0.0 + x is x for every x);
+// float-float: a Dot2 accumulator.
+template <typename V>
+struct Acc;
+template <>
+struct Acc<double> {
+    double a = -0.0;
+    __device__ __forceinline__ void mac(double m, double x) { a += m * x; }
+    __device__ __forceinline__ void mac(double m, float x) { a += m * double(x); }
+    __device__ __forceinline__ double get() const { return a; }
+};
+template <>
+struct Acc<ff> {
+    dot2 d;
+    __device__ __forceinline__ void mac(ff m, ff x) { d.add(m, x); }
+    __device__ __forceinline__ void mac(ff m, float x) { d.add(m, x); }
+    __device__ __forceinline__ ff get() const { return d.value(); }
+};
+
+__device__ __forceinline__ double sum3(double a, double b, double c) { return a + b + c; }
+__device__ __forceinline__ ff sum3(ff a, ff b, ff c)
+{
+    dot2 d;
+    d.add(a);
+    d.add(b);
+    d.add(c);
+    return d.value();
+}
+
+__device__ __forceinline__ double mul(double a, double b) { return a * b; }
+__device__ __forceinline__ ff mul(ff a, ff b)
+{
+    dot2 d;
+    d.add(a, b);
+    return d.value();
+}
+
+// a loaded value as a value of the form
+__device__ __forceinline__ double to_v(double x, double*) { return x; }
+__device__ __forceinline__ double to_v(float x, double*) { return double(x); }
+__device__ __forceinline__ ff to_v(float x, ff*) { return ff{x, 0.0f}; }
+__device__ __forceinline__ ff to_v(ff x, ff*) { return x; }
+
+__device__ __forceinline__ void store_v(double* p, double x) { *p = x; }
+__device__ __forceinline__ void store_v(double* p, ff x) { *p = ff_join(x); }
+
+// the rows of B or D: FP64 through a scalar-load pointer, or the LDS copy
+template <int TAG>
+__device__ __forceinline__ global_ptr<double> mat(const double* p, int dep)
+{
+    return launder<TAG>(p, dep);
+}
+template <int TAG>
+__device__ __forceinline__ const ff* mat(const ff* p, int)
+{
+    return p;
+}
+
 // With SB, a scheduling barrier after each row of M keeps one row of scalar
 // operands live at a time (lower register use, the row's load latency
 // exposed); without it the compiler schedules the row loads freely.
@@ -38,149 +142,110 @@ __device__ __forceinline__ void row_fence()
 
 // out[c] = sum_m M[m * NQ + c] in[m] (c < NOUT): row m of M times in[m],
 // m ascending.
-template <int NQ, bool SB, int NIN, int NOUT, typename P>
-__device__ __forceinline__ void contract_cols(P M, const double (&in)[NIN], double (&out)[NOUT])
+template <int NQ, bool SB, int NIN, int NOUT, typename P, typename X, typename V>
+__device__ __forceinline__ void contract_cols(P M, const X (&in)[NIN], V (&out)[NOUT])
 {
-#pragma unroll
-    for (int c = 0; c < NOUT; ++c) {
-        out[c] = 0.0;
-    }
+    Acc<V> acc[NOUT];
 #pragma unroll
     for (int m = 0; m < NIN; ++m) {
 #pragma unroll
         for (int c = 0; c < NOUT; ++c) {
-            out[c] += M[m * NQ + c] * in[m];
+            acc[c].mac(M[m * NQ + c], in[m]);
         }
         row_fence<SB>();
+    }
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) {
+        out[c] = acc[c].get();
     }
 }
 
 // the same for two inputs sharing the rows of M
-template <int NQ, bool SB, int NIN, int NOUT, typename P>
-__device__ __forceinline__ void contract_cols2(P M, const double (&in0)[NIN], double (&out0)[NOUT],
-                                               const double (&in1)[NIN], double (&out1)[NOUT])
+template <int NQ, bool SB, int NIN, int NOUT, typename P, typename V>
+__device__ __forceinline__ void contract_cols2(P M, const V (&in0)[NIN], V (&out0)[NOUT], const V (&in1)[NIN],
+                                               V (&out1)[NOUT])
 {
-#pragma unroll
-    for (int c = 0; c < NOUT; ++c) {
-        out0[c] = 0.0;
-        out1[c] = 0.0;
-    }
+    Acc<V> acc0[NOUT], acc1[NOUT];
 #pragma unroll
     for (int m = 0; m < NIN; ++m) {
 #pragma unroll
         for (int c = 0; c < NOUT; ++c) {
-            out0[c] += M[m * NQ + c] * in0[m];
-            out1[c] += M[m * NQ + c] * in1[m];
+            acc0[c].mac(M[m * NQ + c], in0[m]);
+            acc1[c].mac(M[m * NQ + c], in1[m]);
         }
         row_fence<SB>();
+    }
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) {
+        out0[c] = acc0[c].get();
+        out1[c] = acc1[c].get();
     }
 }
 
 // out[c] = sum_m M[c * NQ + m] in[m] (c < NOUT): row c of M dotted with in.
-template <int NQ, bool SB, int NIN, int NOUT, typename P>
-__device__ __forceinline__ void contract_rows(P M, const double (&in)[NIN], double (&out)[NOUT])
+template <int NQ, bool SB, int NIN, int NOUT, typename P, typename V>
+__device__ __forceinline__ void contract_rows(P M, const V (&in)[NIN], V (&out)[NOUT])
 {
 #pragma unroll
     for (int c = 0; c < NOUT; ++c) {
-        double t = 0.0;
+        Acc<V> t;
 #pragma unroll
         for (int m = 0; m < NIN; ++m) {
-            t += M[c * NQ + m] * in[m];
+            t.mac(M[c * NQ + m], in[m]);
         }
-        out[c] = t;
+        out[c] = t.get();
         row_fence<SB>();
     }
 }
 
 // the same for three inputs sharing the rows of M
-template <int NQ, bool SB, int NIN, int NOUT, typename P>
-__device__ __forceinline__ void contract_rows3(P M, const double (&in0)[NIN], double (&out0)[NOUT],
-                                               const double (&in1)[NIN], double (&out1)[NOUT],
-                                               const double (&in2)[NIN], double (&out2)[NOUT])
+template <int NQ, bool SB, int NIN, int NOUT, typename P, typename V>
+__device__ __forceinline__ void contract_rows3(P M, const V (&in0)[NIN], V (&out0)[NOUT], const V (&in1)[NIN],
+                                               V (&out1)[NOUT], const V (&in2)[NIN], V (&out2)[NOUT])
 {
 #pragma unroll
     for (int c = 0; c < NOUT; ++c) {
-        double t0 = 0.0, t1 = 0.0, t2 = 0.0;
+        Acc<V> t0, t1, t2;
 #pragma unroll
         for (int m = 0; m < NIN; ++m) {
-            t0 += M[c * NQ + m] * in0[m];
-            t1 += M[c * NQ + m] * in1[m];
-            t2 += M[c * NQ + m] * in2[m];
+            t0.mac(M[c * NQ + m], in0[m]);
+            t1.mac(M[c * NQ + m], in1[m]);
+            t2.mac(M[c * NQ + m], in2[m]);
         }
-        out0[c] = t0;
-        out1[c] = t1;
-        out2[c] = t2;
+        out0[c] = t0.get();
+        out1[c] = t1.get();
+        out2[c] = t2.get();
         row_fence<SB>();
     }
 }
 
-// What a lane loads from HBM before an element's phases start: its u
-// column u(e, :, j, k) (lanes (j, k) < nm^2), and BK1's JxW(e, t, lane) for
-// t < nq (staged into LDS in P1/P2), or BK3's first GPF rows G(e, :, p, q, r)
-// for p < GPF (used in P5; with GIN false they are loaded at the start of P3
-// instead).  bk13t_load issues the loads; the registers are read when the
-// phases get to them, so the loads are in flight meanwhile.
-template <int NM, int NQ, bool BK3, int GPF>
-struct Bk13tIn {
-    static constexpr int nqv = BK3 ? (GPF > 0 ? GPF * 6 : 1) : NQ;
-    double x[NM];
-    double q[nqv];
-};
-
-// ue, Qe: this lane's element's u and JxW/G.  When the lane's element is not
-// active they must still point at a whole readable element of the same
-// order: the loads are unconditional (no branch around each load).
-template <int NM, int NQ, bool BK3, int GPF>
-__device__ __forceinline__ void bk13t_load_g(Bk13tIn<NM, NQ, BK3, GPF>& in, const double* __restrict__ Qe,
-                                             bool in_elem, int l)
+// B (nm x nq) and D (nq x nq) split into float-float in LDS by the whole
+// workgroup (the float-float forms); the caller synchronises afterwards.
+template <int NM, int NQ, bool BK3>
+__device__ __forceinline__ void split_bases(const double* __restrict__ B, const double* __restrict__ D, ff* sB,
+                                            ff* sD, int tid, int nthreads)
 {
-    constexpr int nq2 = NQ * NQ;
-    constexpr int nq3 = nq2 * NQ;
-    if constexpr (BK3 && GPF > 0) {
-        if (in_elem) {
-#pragma unroll
-            for (int p = 0; p < GPF; ++p) {
-#pragma unroll
-                for (int f = 0; f < 6; ++f) {
-                    in.q[p * 6 + f] = Qe[l + f * nq3 + p * nq2];
-                }
-            }
+    for (int x = tid; x < NM * NQ; x += nthreads) {
+        sB[x] = ff_split(B[x]);
+    }
+    if constexpr (BK3) {
+        for (int x = tid; x < NQ * NQ; x += nthreads) {
+            sD[x] = ff_split(D[x]);
         }
     }
 }
 
-template <int NM, int NQ, bool BK3, int GPF, bool GIN = true>
-__device__ __forceinline__ void bk13t_load(Bk13tIn<NM, NQ, BK3, GPF>& in, const double* __restrict__ ue,
-                                           const double* __restrict__ Qe, bool in_elem, int l)
-{
-    constexpr int nm2 = NM * NM;
-    constexpr int nq2 = NQ * NQ;
-    constexpr int nq3 = nq2 * NQ;
-    if (in_elem && l < nm2) {
-#pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            in.x[i] = ue[i * nm2 + l];
-        }
-    }
-    if constexpr (!BK3) {
-        if (in_elem) {
-#pragma unroll
-            for (int t = 0; t < NQ; ++t) {
-                in.q[t] = Qe[t * nq2 + l];
-            }
-        }
-    } else if constexpr (GIN) {
-        bk13t_load_g(in, Qe, in_elem, l);
-    }
-    (void)nq3;
-}
-
-// One workgroup's pass over up to epb elements, from the loads in `in`.  Qe
-// and ve are this lane's element's JxW/G and v (Qe readable as in
-// bk13t_load; v is stored only when active).  With `next`, the loads of the
-// lane's next element (next_ue, next_Qe) are issued into `in` at the start
-// of P7, when the registers of `in` are free, so that they overlap P7-P9 of
-// this element.  smem holds epb * NB boxes.
+// One workgroup's pass over up to epb elements.  ue, Qe and ve are this
+// lane's element's u, JxW/G and v.  When !active they must still point at a
+// whole readable element of the same order: the loads are unconditional
+// (no branch around each load) and only the store is guarded.  MB, MD are
+// the rows of B and D (FP64 device pointers, or the float-float LDS copy of
+// split_bases for the float-float forms); smem holds epb * NB boxes.
+//
+// The lane's u column u(e, :, j, k) (lanes (j, k) < nm^2), and BK1's
+// JxW(e, t, lane) for t < nq or BK3's first GPF rows G(e, :, p, q, r),
+// p < GPF, are loaded at the start, so that they are in flight through the
+// first phases.
 //
 // NB = 3: boxes A, B, C as in the phase table of bk13t.hip.
 // NB = 2: boxes A and B only (two thirds of the LDS, so more workgroups per
@@ -190,38 +255,71 @@ __device__ __forceinline__ void bk13t_load(Bk13tIn<NM, NQ, BK3, GPF>& in, const 
 //        has just read; P4 stores qs to box A after every lane has read U.
 //        P5 then works in place on qs (box A) and qt (box B), and P6 writes
 //        Wr and Ws after a barrier.
-// GPF > 0 (BK3): the first GPF rows of G come with `in`, and P5 keeps GPF
-// rows in flight (row p + GPF is loaded when row p is used).  With GPF = 0,
-// P5 loads one row ahead.
-template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF, bool GIN = true>
-__device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const double* __restrict__ D,
-                                          Bk13tIn<NM, NQ, BK3, GPF>& in, const double* __restrict__ Qe,
-                                          double* __restrict__ ve, bool active, int slot, int l, double* smem,
-                                          bool next, const double* __restrict__ next_ue,
-                                          const double* __restrict__ next_Qe, int dep = 0)
+// GPF > 0 (BK3): P5 keeps GPF rows of G in flight (row p + GPF is loaded
+// when row p is used).  With GPF = 0, P5 loads one row ahead.
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB = 3, int GPF = 0, int FORM = BKX_FORM_F64,
+          typename MT = double>
+__device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* __restrict__ D,
+                                           const typename Form<FORM>::TS* __restrict__ Qe,
+                                           const typename Form<FORM>::TS* __restrict__ ue,
+                                           double* __restrict__ ve, bool active, int slot, int l, double* smem,
+                                           int dep = 0)
 {
     static_assert(NB == 2 || NB == 3, "two or three LDS boxes per element");
     static_assert(GPF >= 0 && GPF <= NQ, "G prefetch depth");
+    using V = typename Form<FORM>::V;
+    using L = typename Form<FORM>::L;
+    static_assert(sizeof(V) == sizeof(double), "an LDS box holds 8-byte values");
     using S = Bk13tShape<NM, NQ, DIV>;
     constexpr int nq2 = S::nq2;
     constexpr int R = S::row;
     const bool in_elem = slot < S::epb;
     const int sl = in_elem ? slot : 0;
-    double* bA = smem + (sl * NB + 0) * S::box;
-    double* bB = smem + (sl * NB + 1) * S::box;
+    V* bA = reinterpret_cast<V*>(smem) + (sl * NB + 0) * S::box;
+    V* bB = reinterpret_cast<V*>(smem) + (sl * NB + 1) * S::box;
     // box C: the third box, or (NB = 2) the box holding qt and rt, which is B
-    double* bC = smem + (sl * NB + (NB == 3 ? 2 : 1)) * S::box;
+    V* bC = reinterpret_cast<V*>(smem) + (sl * NB + (NB == 3 ? 2 : 1)) * S::box;
     // box holding qs and rs: B (NB = 3) or A (NB = 2)
-    double* bS = NB == 3 ? bB : bA;
+    V* bS = NB == 3 ? bB : bA;
     auto X = [](int a, int b, int c) { return (a * NQ + b) * R + c; };
     // lane (a, b) of an NA x NB plane, when l < NA * NB
     const int l_m = l / NM, l_m2 = l - (l / NM) * NM;    // (l / NM, l % NM)
     const int l_q = l / NQ, l_q2 = l - (l / NQ) * NQ;    // (l / NQ, l % NQ)
+    L* const tagL = nullptr;   // selects load_as overloads
+    V* const tagV = nullptr;
+
+    // ---- loads ------------------------------------------------------------
+    L xin[NM];
+    constexpr int nqv = BK3 ? (GPF > 0 ? GPF * 6 : 1) : NQ;
+    L qin[nqv];
+    if (in_elem && l < S::nm2) {
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            xin[i] = load_as(ue + i * S::nm2 + l, tagL);
+        }
+    }
+    const auto* ge = Qe + l;   // BK3: G(e, f, p, q, r) at f = p = 0, (q, r) = lane
+    if (in_elem) {
+        if constexpr (!BK3) {
+#pragma unroll
+            for (int t = 0; t < NQ; ++t) {
+                qin[t] = load_as(Qe + t * nq2 + l, tagL);
+            }
+        } else if constexpr (GPF > 0) {
+#pragma unroll
+            for (int p = 0; p < GPF; ++p) {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    qin[p * 6 + f] = load_as(ge + f * S::nq3 + p * nq2, tagL);
+                }
+            }
+        }
+    }
 
     // ---- P1: lane (j,k), contract i -> p ---------------------------------
     if (in_elem && l < S::nm2) {
-        double a[NQ];
-        contract_cols<NQ, SB>(launder<1>(B, dep), in.x, a);
+        V a[NQ];
+        contract_cols<NQ, SB>(mat<1>(B, dep), xin, a);
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {
             bA[X(p, l_m, l_m2)] = a[p];
@@ -234,7 +332,7 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
 #pragma unroll
             for (int t = 0; t < NQ; ++t) {
                 const int x = t * nq2 + l;
-                bC[x + x / NQ] = in.q[t];
+                bC[x + x / NQ] = to_v(qin[t], tagV);
             }
         }
     }
@@ -242,15 +340,15 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
 
     // ---- P2: lane (p,k), contract j -> q ---------------------------------
     {
-        double b[NQ];
+        V b[NQ];
         const bool p2 = in_elem && l < NQ * NM;
         if (p2) {
-            double y[NM];
+            V y[NM];
 #pragma unroll
             for (int j = 0; j < NM; ++j) {
                 y[j] = bA[X(l_m, j, l_m2)];
             }
-            contract_cols<NQ, SB>(launder<2>(B, dep), y, b);
+            contract_cols<NQ, SB>(mat<2>(B, dep), y, b);
         }
         if constexpr (!BK3 && NB == 2) {
             __syncthreads();
@@ -258,7 +356,7 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
 #pragma unroll
                 for (int t = 0; t < NQ; ++t) {
                     const int x = t * nq2 + l;
-                    bA[x + x / NQ] = in.q[t];
+                    bA[x + x / NQ] = to_v(qin[t], tagV);
                 }
             }
         }
@@ -271,28 +369,24 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
     }
     __syncthreads();
     // BK1: box holding JxW in P3
-    const double* bJ = NB == 3 ? bC : bA;
+    const V* bJ = NB == 3 ? bC : bA;
 
     // ---- P3: lane (p,q), contract k -> r ---------------------------------
-    double W[NQ];
-    const double* ge = Qe + l;   // BK3: G(e, f, p, q, r) at f = p = 0, (q, r) = lane
-    if constexpr (!GIN) {
-        bk13t_load_g(in, Qe, in_elem, l);
-    }
+    V W[NQ];
     if (in_elem) {
-        double y[NM], U[NQ];
+        V y[NM], U[NQ];
 #pragma unroll
         for (int k = 0; k < NM; ++k) {
             y[k] = bB[X(l_q, l_q2, k)];
         }
-        contract_cols<NQ, SB>(launder<3>(B, dep), y, U);
+        contract_cols<NQ, SB>(mat<3>(B, dep), y, U);
         if constexpr (!BK3) {
 #pragma unroll
             for (int r = 0; r < NQ; ++r) {
-                W[r] = U[r] * bJ[X(l_q, l_q2, r)];
+                W[r] = mul(U[r], bJ[X(l_q, l_q2, r)]);
             }
         } else {
-            contract_cols<NQ, SB>(launder<4>(D, dep), U, W);   // qt, held in W
+            contract_cols<NQ, SB>(mat<4>(D, dep), U, W);   // qt, held in W
 #pragma unroll
             for (int r = 0; r < NQ; ++r) {
                 bA[X(l_q, l_q2, r)] = U[r];
@@ -314,18 +408,18 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
             }
         }
         // ---- P4: qs in lane (p,r) col q; qr in lane (q,r) col p ----------
-        double qr[NQ];
+        V qr[NQ];
         {
-            double qs[NQ];
+            V qs[NQ];
             if (in_elem) {
-                double y[NQ], z[NQ];
+                V y[NQ], z[NQ];
                 if constexpr (NB == 3) {
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) {
                         y[q] = bA[X(l_q, q, l_q2)];      // lane (p, r), col q
                         z[q] = bA[X(q, l_q, l_q2)];      // lane (q, r), col p
                     }
-                    contract_cols2<NQ, SB>(launder<5>(D, dep), y, qs, z, qr);
+                    contract_cols2<NQ, SB>(mat<5>(D, dep), y, qs, z, qr);
                 } else {
                     // one input at a time (fewer registers live; D's rows
                     // are read twice from the scalar cache)
@@ -333,12 +427,12 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
                     for (int q = 0; q < NQ; ++q) {
                         y[q] = bA[X(l_q, q, l_q2)];
                     }
-                    contract_cols<NQ, SB>(launder<5>(D, dep), y, qs);
+                    contract_cols<NQ, SB>(mat<5>(D, dep), y, qs);
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) {
                         z[q] = bA[X(q, l_q, l_q2)];
                     }
-                    contract_cols<NQ, SB>(launder<6>(D, dep), z, qr);
+                    contract_cols<NQ, SB>(mat<6>(D, dep), z, qr);
                 }
             }
             if constexpr (NB == 2) {
@@ -353,18 +447,18 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
         }
         __syncthreads();
         // ---- P5: fluxes in lane (q,r) col p ---------------------------------
-        double rr[NQ];
+        V rr[NQ];
         if (in_elem) {
-            double gn[6];
+            L gn[6];
             if constexpr (GPF == 0) {
 #pragma unroll
                 for (int f = 0; f < 6; ++f) {
-                    gn[f] = ge[f * S::nq3];
+                    gn[f] = load_as(ge + f * S::nq3, tagL);
                 }
             }
 #pragma unroll
             for (int p = 0; p < NQ; ++p) {
-                double g[6];
+                L g[6];
                 if constexpr (GPF == 0) {
 #pragma unroll
                     for (int f = 0; f < 6; ++f) {
@@ -373,27 +467,38 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
                     if (p + 1 < NQ) {
 #pragma unroll
                         for (int f = 0; f < 6; ++f) {
-                            gn[f] = ge[f * S::nq3 + (p + 1) * nq2];
+                            gn[f] = load_as(ge + f * S::nq3 + (p + 1) * nq2, tagL);
                         }
                     }
                 } else {
 #pragma unroll
                     for (int f = 0; f < 6; ++f) {
-                        g[f] = in.q[(p % GPF) * 6 + f];
+                        g[f] = qin[(p % GPF) * 6 + f];
                     }
                     if (p + GPF < NQ) {
 #pragma unroll
                         for (int f = 0; f < 6; ++f) {
-                            in.q[(p % GPF) * 6 + f] = ge[f * S::nq3 + (p + GPF) * nq2];
+                            qin[(p % GPF) * 6 + f] = load_as(ge + f * S::nq3 + (p + GPF) * nq2, tagL);
                         }
                     }
                 }
                 const int x = X(p, l_q, l_q2);
-                const double qs = bS[x];
-                const double qt = bC[x];
-                rr[p] = g[0] * qr[p] + g[1] * qs + g[2] * qt;
-                bS[x] = g[1] * qr[p] + g[3] * qs + g[4] * qt;
-                bC[x] = g[2] * qr[p] + g[4] * qs + g[5] * qt;
+                const V qs = bS[x];
+                const V qt = bC[x];
+                // (rr, rs, rt) = G (qr, qs, qt), G symmetric: rr, rs, rt, ss, st, tt
+                Acc<V> a0, a1, a2;
+                a0.mac(qr[p], g[0]);
+                a0.mac(qs, g[1]);
+                a0.mac(qt, g[2]);
+                a1.mac(qr[p], g[1]);
+                a1.mac(qs, g[3]);
+                a1.mac(qt, g[4]);
+                a2.mac(qr[p], g[2]);
+                a2.mac(qs, g[4]);
+                a2.mac(qt, g[5]);
+                rr[p] = a0.get();
+                bS[x] = a1.get();
+                bC[x] = a2.get();
                 if constexpr (GPF > 0) {
                     // keep the rows' loads where they are written: GPF rows
                     // in flight, not all of G hoisted to the top of P5
@@ -403,11 +508,11 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
         }
         __syncthreads();
         // ---- P6: transposed gradient ---------------------------------------
-        double Ws[NQ], Wt[NQ];
+        V Ws[NQ], Wt[NQ];
         {
-            double Wr[NQ];
+            V Wr[NQ];
             if (in_elem) {
-                double y[NQ], z[NQ];
+                V y[NQ], z[NQ];
                 // Wr: lane (q, r), col p;  Ws: lane (p, r), col q;  Wt: lane (p, q), col r
                 if constexpr (NB == 3) {
 #pragma unroll
@@ -415,19 +520,19 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
                         y[q] = bS[X(l_q, q, l_q2)];      // rs, lane (p, r), col q
                         z[q] = bC[X(l_q, l_q2, q)];      // rt, lane (p, q), col r
                     }
-                    contract_rows3<NQ, SB>(launder<7>(D, dep), rr, Wr, y, Ws, z, Wt);
+                    contract_rows3<NQ, SB>(mat<7>(D, dep), rr, Wr, y, Ws, z, Wt);
                 } else {
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) {
                         z[q] = bC[X(l_q, l_q2, q)];
                     }
-                    contract_rows<NQ, SB>(launder<7>(D, dep), z, Wt);
+                    contract_rows<NQ, SB>(mat<7>(D, dep), z, Wt);
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) {
                         y[q] = bS[X(l_q, q, l_q2)];
                     }
-                    contract_rows<NQ, SB>(launder<8>(D, dep), y, Ws);
-                    contract_rows<NQ, SB>(launder<9>(D, dep), rr, Wr);
+                    contract_rows<NQ, SB>(mat<8>(D, dep), y, Ws);
+                    contract_rows<NQ, SB>(mat<9>(D, dep), rr, Wr);
                 }
             }
             if constexpr (NB == 2) {
@@ -454,18 +559,15 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
 #pragma unroll
             for (int r = 0; r < NQ; ++r) {
                 const int x = X(l_q, l_q2, r);
-                W[r] = Wt[r] + bA[x] + bB[x];
+                W[r] = sum3(Wt[r], bA[x], bB[x]);
             }
         }
     }
 
     // ---- P7: lane (p,q), contract r -> k ---------------------------------
-    if (next) {
-        bk13t_load<NM, NQ, BK3, GPF, GIN>(in, next_ue, next_Qe, in_elem, l);
-    }
     if (in_elem) {
-        double Y[NM];
-        contract_rows<NQ, SB>(launder<10>(B, dep), W, Y);
+        V Y[NM];
+        contract_rows<NQ, SB>(mat<10>(B, dep), W, Y);
 #pragma unroll
         for (int k = 0; k < NM; ++k) {
             bA[X(l_q, l_q2, k)] = Y[k];
@@ -475,12 +577,12 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
 
     // ---- P8: lane (p,k), contract q -> j ---------------------------------
     if (in_elem && l < NQ * NM) {
-        double y[NQ], Z[NM];
+        V y[NQ], Z[NM];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             y[q] = bA[X(l_m, q, l_m2)];
         }
-        contract_rows<NQ, SB>(launder<11>(B, dep), y, Z);
+        contract_rows<NQ, SB>(mat<11>(B, dep), y, Z);
 #pragma unroll
         for (int j = 0; j < NM; ++j) {
             bB[X(l_m, j, l_m2)] = Z[j];
@@ -490,31 +592,17 @@ __device__ __forceinline__ void bk13t_run(const double* __restrict__ B, const do
 
     // ---- P9: lane (j,k), contract p -> i, store --------------------------
     if (active && l < S::nm2) {
-        double z[NQ], o[NM];
+        V z[NQ], o[NM];
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {
             z[p] = bB[X(p, l_m, l_m2)];
         }
-        contract_rows<NQ, SB>(launder<12>(B, dep), z, o);
+        contract_rows<NQ, SB>(mat<12>(B, dep), z, o);
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            ve[i * S::nm2 + l] = o[i];
+            store_v(ve + i * S::nm2 + l, o[i]);
         }
     }
-}
-
-// One workgroup's pass over up to epb elements: bk13t_load, then bk13t_run.
-// ue, Qe and ve are this lane's element's u, JxW/G and v (ue and Qe
-// readable as in bk13t_load when !active).
-template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB = 3, int GPF = 0>
-__device__ __forceinline__ void bk13t_body(const double* __restrict__ B, const double* __restrict__ D,
-                                           const double* __restrict__ Qe, const double* __restrict__ ue,
-                                           double* __restrict__ ve, bool active, int slot, int l, double* smem,
-                                           int dep = 0)
-{
-    Bk13tIn<NM, NQ, BK3, GPF> in;
-    bk13t_load(in, ue, Qe, slot < Bk13tShape<NM, NQ, DIV>::epb, l);
-    bk13t_run<NM, NQ, BK3, DIV, SB, NB, GPF>(B, D, in, Qe, ve, active, slot, l, smem, false, nullptr, nullptr, dep);
 }
 
 }  // namespace bk13t

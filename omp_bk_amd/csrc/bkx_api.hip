@@ -31,6 +31,11 @@ int bk5_apply_f32c(int p, size_t nelmt, const double* D, const float* G, const f
                    hipStream_t s);
 int bk5_apply_f32w(int variant, int p, size_t nelmt, const double* D, const float* G, const float* u, double* v,
                    hipStream_t s);
+int bk13_red_variants(int form);
+int bk1_apply_red(int form, int variant, int p, size_t nelmt, const double* B, const void* JxW, const void* u,
+                  double* v, hipStream_t s);
+int bk3_apply_red(int form, int variant, int p, size_t nelmt, const double* B, const double* D, const void* G,
+                  const void* u, double* v, hipStream_t s);
 }  // namespace bkx
 
 namespace {
@@ -44,7 +49,7 @@ hipStream_t as_stream(bkx_stream_t s) { return reinterpret_cast<hipStream_t>(s);
 // E = 65536, BK1 and BK3 over 32768 elements); the autotuner
 // overwrites them at run time.
 constexpr int kOps = 6;
-constexpr int kForms = 4;
+constexpr int kForms = 5;
 constexpr int kOrders = 9;
 
 struct VariantTable {
@@ -63,11 +68,23 @@ struct VariantTable {
         static const int bk5_f32w[kOrders] = {0, 17, 17, 17, 17, 17, 17, 17, 0};
         static const int bk3_f64[kOrders] = {0, 1, 8, 6, 7, 6, 8, 6, 7};
         static const int bk1_f64[kOrders] = {0, 1, 5, 5, 5, 4, 5, 4, 5};
+        static const int bk3_f32w[kOrders] = {0, 1, 1, 2, 2, 0, 0, 2, 2};
+        static const int bk3_f32c[kOrders] = {0, 2, 2, 2, 0, 2, 1, 0, 0};
+        static const int bk3_ff[kOrders] = {0, 2, 2, 2, 2, 2, 0, 0, 1};
+        static const int bk1_f32w[kOrders] = {0, 1, 2, 2, 2, 2, 1, 2, 2};
+        static const int bk1_f32c[kOrders] = {0, 2, 2, 0, 0, 0, 1, 0, 1};
+        static const int bk1_ff[kOrders] = {0, 2, 2, 0, 0, 0, 1, 0, 0};
         for (int p = 0; p < kOrders; ++p) {
             v[5][BKX_FORM_F64][p].store(bk5_f64[p], std::memory_order_relaxed);
             v[5][BKX_FORM_F32W][p].store(bk5_f32w[p], std::memory_order_relaxed);
             v[3][BKX_FORM_F64][p].store(bk3_f64[p], std::memory_order_relaxed);
             v[1][BKX_FORM_F64][p].store(bk1_f64[p], std::memory_order_relaxed);
+            v[3][BKX_FORM_F32W][p].store(bk3_f32w[p], std::memory_order_relaxed);
+            v[3][BKX_FORM_F32C][p].store(bk3_f32c[p], std::memory_order_relaxed);
+            v[3][BKX_FORM_FF][p].store(bk3_ff[p], std::memory_order_relaxed);
+            v[1][BKX_FORM_F32W][p].store(bk1_f32w[p], std::memory_order_relaxed);
+            v[1][BKX_FORM_F32C][p].store(bk1_f32c[p], std::memory_order_relaxed);
+            v[1][BKX_FORM_FF][p].store(bk1_ff[p], std::memory_order_relaxed);
         }
     }
 };
@@ -113,6 +130,9 @@ int bkx_tune_count(int op, int form)
 {
     if (op == 5 && (form == BKX_FORM_F64 || form == BKX_FORM_F32W)) {
         return bkx::bk5_variants();
+    }
+    if ((op == 1 || op == 3) && (form == BKX_FORM_F32W || form == BKX_FORM_F32C || form == BKX_FORM_FF)) {
+        return bkx::bk13_red_variants(form);   // bk13t_red.hip
     }
     if ((op == 1 || op == 3) && form == BKX_FORM_F64) {
         // transposing mapping (bk13t.hip): 0 all elements per workgroup,
@@ -188,6 +208,80 @@ int bkx_bk3_apply_f64(int p, size_t nelmt, const double* basis, const double* db
         return bkx::bk3_apply_f64(p, nelmt, basis, dbasis, G, in, out, as_stream(stream));
     }
     return bkx::bk3t_apply_f64(var, p, nelmt, basis, dbasis, G, in, out, as_stream(stream));
+}
+
+}  // extern "C"
+
+namespace {
+
+int bk1_red(int form, int p, size_t nelmt, const double* basis, const void* JxW, const void* in, double* out,
+            bkx_stream_t stream)
+{
+    if (!order_ok(1, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !JxW || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk1_apply_red(form, variant_of(1, form, p), p, nelmt, basis, JxW, in, out, as_stream(stream));
+}
+
+int bk3_red(int form, int p, size_t nelmt, const double* basis, const double* dbasis, const void* G, const void* in,
+            double* out, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_apply_red(form, variant_of(3, form, p), p, nelmt, basis, dbasis, G, in, out, as_stream(stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+int bkx_bk1_apply_f32w(int p, size_t nelmt, const double* basis, const float* JxW, const float* in, double* out,
+                       bkx_stream_t stream)
+{
+    return bk1_red(BKX_FORM_F32W, p, nelmt, basis, JxW, in, out, stream);
+}
+
+int bkx_bk1_apply_f32c(int p, size_t nelmt, const double* basis, const float* JxW, const float* in, double* out,
+                       bkx_stream_t stream)
+{
+    return bk1_red(BKX_FORM_F32C, p, nelmt, basis, JxW, in, out, stream);
+}
+
+int bkx_bk1_apply_ff(int p, size_t nelmt, const double* basis, const double* JxW, const double* in, double* out,
+                     bkx_stream_t stream)
+{
+    return bk1_red(BKX_FORM_FF, p, nelmt, basis, JxW, in, out, stream);
+}
+
+int bkx_bk3_apply_f32w(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                       const float* in, double* out, bkx_stream_t stream)
+{
+    return bk3_red(BKX_FORM_F32W, p, nelmt, basis, dbasis, G, in, out, stream);
+}
+
+int bkx_bk3_apply_f32c(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                       const float* in, double* out, bkx_stream_t stream)
+{
+    return bk3_red(BKX_FORM_F32C, p, nelmt, basis, dbasis, G, in, out, stream);
+}
+
+int bkx_bk3_apply_ff(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                     const double* in, double* out, bkx_stream_t stream)
+{
+    return bk3_red(BKX_FORM_FF, p, nelmt, basis, dbasis, G, in, out, stream);
 }
 
 int bkx_bk5_apply_f64(int p, size_t nelmt, const double* dbasis, const double* G, const double* in, double* out,

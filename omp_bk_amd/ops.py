@@ -43,42 +43,71 @@ def _order_ok(op, p):
         raise ValueError("unsupported polynomial order p = %d (BK%d supports %d..%d)" % (p, op, lo, hi))
 
 
-def bk1(p, basis, JxW, u, out=None):
-    """BK1 mass operator v = B^T diag(JxW) B u per element (FP64)."""
+# form: (entry-point suffix, storage dtype of u and JxW/G); v is FP64 in all
+_BK13_FORMS = {
+    "f64": ("f64", torch.float64),
+    "ff": ("ff", torch.float64),
+    "f32c": ("f32c", torch.float32),
+    "f32w": ("f32w", torch.float32),
+}
+
+def _bk13_form(op, form):
+    if form not in _BK13_FORMS:
+        raise ValueError("unknown BK%d form %r (one of %s)" % (op, form, ", ".join(_BK13_FORMS)))
+    suffix, sdt = _BK13_FORMS[form]
+    return "bkx_bk%d_apply_%s" % (op, suffix), sdt
+
+
+def bk1(p, basis, JxW, u, out=None, form="f64"):
+    """BK1 mass operator v = B^T diag(JxW) B u per element.
+
+    form "f64": u, JxW, v in FP64 (the drop-in);
+         "ff": FP64 storage, float-float arithmetic (u and JxW split into
+               hi + lo when loaded; the reference's BK1_ff,
+               openmp/BK1_ff.cpp:73-723), v FP64;
+         "f32c": u, JxW in FP32, float-float compensated arithmetic, v FP64;
+         "f32w": u, JxW in FP32, FP64 arithmetic, v FP64."""
     _order_ok(1, p)
+    entry, sdt = _bk13_form(1, form)
     nm, nq = p + 1, p + 2
     if u.numel() % nm ** 3:
         raise ValueError("u must hold whole elements of %d values" % nm ** 3)
     ne = u.numel() // nm ** 3
     _check("basis", basis, nm * nq)
-    _check("JxW", JxW, ne * nq ** 3)
-    _check("u", u, ne * nm ** 3)
+    _check("JxW", JxW, ne * nq ** 3, sdt)
+    _check("u", u, ne * nm ** 3, sdt)
     if out is None:
-        out = torch.empty_like(u)
+        out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nm ** 3)
     with torch.cuda.device(u.device):
-        _lib.call("bkx_bk1_apply_f64", p, ne, basis.data_ptr(), JxW.data_ptr(), u.data_ptr(), out.data_ptr(),
-                  _stream(u))
+        _lib.call(entry, p, ne, basis.data_ptr(), JxW.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
     return out
 
 
-def bk3(p, basis, dbasis, G, u, out=None):
-    """BK3 Poisson operator v = B^T D^T G D B u per element (FP64)."""
+def bk3(p, basis, dbasis, G, u, out=None, form="f64"):
+    """BK3 Poisson operator v = B^T D^T G D B u per element.
+
+    form "f64": u, G, v in FP64 (the drop-in);
+         "ff": FP64 storage, float-float arithmetic (u and G split into
+               hi + lo when loaded), v FP64;
+         "f32c": u, G in FP32, float-float compensated arithmetic, v FP64;
+         "f32w": u, G in FP32, FP64 arithmetic, v FP64."""
     _order_ok(3, p)
+    entry, sdt = _bk13_form(3, form)
     nm, nq = p + 1, p + 2
     if u.numel() % nm ** 3:
         raise ValueError("u must hold whole elements of %d values" % nm ** 3)
     ne = u.numel() // nm ** 3
     _check("basis", basis, nm * nq)
     _check("dbasis", dbasis, nq * nq)
-    _check("G", G, ne * 6 * nq ** 3)
-    _check("u", u, ne * nm ** 3)
+    _check("G", G, ne * 6 * nq ** 3, sdt)
+    _check("u", u, ne * nm ** 3, sdt)
     if out is None:
-        out = torch.empty_like(u)
+        out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nm ** 3)
     with torch.cuda.device(u.device):
-        _lib.call("bkx_bk3_apply_f64", p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), u.data_ptr(),
-                  out.data_ptr(), _stream(u))
+        _lib.call(entry, p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(),
+                  _stream(u))
     return out
 
 

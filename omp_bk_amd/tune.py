@@ -24,12 +24,12 @@ from . import _lib
 from . import geometry as geo
 from . import ops
 
-FORMS = {"f64": 0, "f32c": 1, "f32w": 2, "bf16s": 3}
+FORMS = {"f64": 0, "f32c": 1, "f32w": 2, "bf16s": 3, "ff": 4}
 
 
 def _workload(op, form, p, nelmt, device):
     torch.manual_seed(p)
-    dt = torch.float64 if form == "f64" else torch.float32
+    dt = torch.float64 if form in ("f64", "ff") else torch.float32
     if op == 5:
         n = p + 1
         D = torch.as_tensor(geo.bk5_basis(p), device=device)
@@ -38,12 +38,12 @@ def _workload(op, form, p, nelmt, device):
         return lambda out: ops.bk5(p, D, G, u, out=out, form=form), nelmt * n ** 3
     n = p + 1
     B, D = (torch.as_tensor(a, device=device) for a in geo.bk13_bases(p))
-    u = torch.rand(nelmt * n ** 3, dtype=torch.float64, device=device) * 2 - 1
+    u = (torch.rand(nelmt * n ** 3, dtype=torch.float64, device=device) * 2 - 1).to(dt)
     if op == 3:
-        G = geo.geometric_factors(3, p, nelmt, shape=_shape(nelmt), device=device).reshape(-1)
-        return lambda out: ops.bk3(p, B, D, G, u, out=out), nelmt * n ** 3
-    J = geo.geometric_factors(1, p, nelmt, shape=_shape(nelmt), device=device).reshape(-1)
-    return lambda out: ops.bk1(p, B, J, u, out=out), nelmt * n ** 3
+        G = geo.geometric_factors(3, p, nelmt, shape=_shape(nelmt), device=device).reshape(-1).to(dt)
+        return lambda out: ops.bk3(p, B, D, G, u, out=out, form=form), nelmt * n ** 3
+    J = geo.geometric_factors(1, p, nelmt, shape=_shape(nelmt), device=device).reshape(-1).to(dt)
+    return lambda out: ops.bk1(p, B, J, u, out=out, form=form), nelmt * n ** 3
 
 
 def _shape(nelmt):
