@@ -64,8 +64,8 @@ struct VariantTable {
             }
         }
         //                            p:   0   1   2   3   4   5   6   7   8
-        static const int bk5_f64[kOrders] = {0, 20, 1, 21, 15, 17, 20, 4, 0};
-        static const int bk5_f32w[kOrders] = {0, 17, 17, 17, 17, 17, 17, 17, 0};
+        static const int bk5_f64[kOrders] = {0, 4, 26, 20, 34, 38, 38, 36, 0};
+        static const int bk5_f32w[kOrders] = {0, 28, 33, 15, 23, 39, 39, 40, 0};
         static const int bk3_f64[kOrders] = {0, 1, 8, 6, 7, 6, 8, 6, 7};
         static const int bk1_f64[kOrders] = {0, 1, 5, 5, 5, 4, 5, 4, 5};
         static const int bk3_f32w[kOrders] = {0, 1, 1, 2, 2, 0, 0, 2, 2};
