@@ -38,9 +38,15 @@ namespace {
 using bk13t::Bk13tShape;
 using bk13t::bk13t_body;
 
-template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF>
+// Occupancy target (wavefronts per SIMD) for GPF rows of G held in
+// registers: none without prefetch, 4 for up to three rows, 3 for up to six,
+// else 2 (a whole element's G, 6 nq values per lane, is loaded with u and
+// in flight through the first four phases).
+constexpr int bk13t_waves(int gpf) { return gpf == 0 ? 1 : (gpf <= 3 ? 4 : (gpf <= 6 ? 3 : 2)); }
+
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF, int NT>
 __global__ __launch_bounds__((Bk13tShape<NM, NQ, DIV>::threads))
-__attribute__((amdgpu_waves_per_eu(GPF > 0 ? 4 : 1))) void bk13t_kernel(
+__attribute__((amdgpu_waves_per_eu(bk13t_waves(GPF)))) void bk13t_kernel(
     size_t nelmt, const double* __restrict__ B, const double* __restrict__ D, const double* __restrict__ Q,
     const double* __restrict__ u, double* __restrict__ v)
 {
@@ -52,12 +58,80 @@ __attribute__((amdgpu_waves_per_eu(GPF > 0 ? 4 : 1))) void bk13t_kernel(
     const bool active = slot < S::epb && e < nelmt;
     const size_t ee = active ? e : 0;
     constexpr size_t qper = BK3 ? 6 * S::nq3 : S::nq3;
-    bk13t_body<NM, NQ, BK3, DIV, SB, NB, GPF>(B, D, Q + ee * qper, u + ee * S::nm3, v + ee * S::nm3, active, slot, l, s_box);
+    bk13t_body<NM, NQ, BK3, DIV, SB, NB, GPF, BKX_FORM_F64, double, NT>(B, D, Q + ee * qper, u + ee * S::nm3,
+                                                                       v + ee * S::nm3, active, slot, l, s_box);
+}
+
+// Persistent BK3 (PIPE): workgroup w processes the element groups
+// w, w + W, w + 2W, ... (W = the number of resident workgroups), each
+// group's first loads issued during the previous group's flux phase.  The
+// loads past the last group re-read the current one (never used), so that
+// every path issues the same memory operations.  Every workgroup leaves the
+// loop after its last group; the barriers inside the phases are reached by
+// all of its threads on every pass (the group loop is uniform).
+template <int NM, int NQ, int DIV, int GPF, int NT, int MINW>
+__global__ __launch_bounds__((Bk13tShape<NM, NQ, DIV>::threads))
+__attribute__((amdgpu_waves_per_eu(MINW))) void bk3p_kernel(
+    size_t nelmt, const double* __restrict__ B, const double* __restrict__ D, const double* __restrict__ Q,
+    const double* __restrict__ u, double* __restrict__ v)
+{
+    using S = Bk13tShape<NM, NQ, DIV>;
+    __shared__ double s_box[S::epb * 2 * S::box];
+    const int slot = threadIdx.x / S::nq2;
+    const int l = threadIdx.x - slot * S::nq2;
+    const size_t ngroups = (nelmt + S::epb - 1) / S::epb;
+    size_t g = blockIdx.x;
+    if (g >= ngroups) {
+        return;
+    }
+    constexpr size_t qper = 6 * S::nq3;
+    // element of this thread in group grp (element 0 where there is none)
+    auto elem = [&](size_t grp) {
+        const size_t e = grp * S::epb + slot;
+        return slot < S::epb && e < nelmt ? e : size_t(0);
+    };
+    size_t ee = elem(g);
+    double xin[NM];
+    double qin[bk13t::bk13t_nqv(true, GPF, NQ)];
+    bk13t::bk13t_load<NM, NQ, true, DIV, GPF, BKX_FORM_F64, NT, true>(Q + ee * qper, u + ee * S::nm3, slot, l, xin, qin);
+    for (;;) {
+        const size_t e = g * S::epb + slot;
+        const bool active = slot < S::epb && e < nelmt;
+        const bool has_next = g + gridDim.x < ngroups;
+        const size_t gn = has_next ? g + gridDim.x : g;
+        const size_t en = elem(gn);
+        bk13t::bk13t_phases<NM, NQ, true, DIV, false, 2, GPF, BKX_FORM_F64, double, NT, true>(
+            B, D, Q + ee * qper, v + ee * S::nm3, active, slot, l, s_box, int(g), xin, qin, Q + en * qper,
+            u + en * S::nm3);
+        if (!has_next) {
+            break;
+        }
+        g = gn;
+        ee = en;
+    }
 }
 
 }  // namespace
 
-template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF = 0>
+// Launches the persistent BK3 kernel with at most as many workgroups as are
+// resident at once.
+template <int NM, int NQ, int DIV, int GPF, int NT, int MINW = 3>
+int launch_bk3p(size_t nelmt, const double* B, const double* D, const double* Q, const double* u, double* v,
+                hipStream_t s)
+{
+    using S = Bk13tShape<NM, NQ, DIV>;
+    const unsigned resident = resident_blocks<bk3p_kernel<NM, NQ, DIV, GPF, NT, MINW>>(S::threads);
+    if (resident == 0) {
+        return BKX_ERROR_LAUNCH;
+    }
+    const size_t ngroups = (nelmt + S::epb - 1) / S::epb;
+    const size_t grid = ngroups < resident ? ngroups : resident;
+    hipLaunchKernelGGL((bk3p_kernel<NM, NQ, DIV, GPF, NT, MINW>), dim3(unsigned(grid)), dim3(S::threads), 0, s, nelmt, B, D,
+                       Q, u, v);
+    return launch_status();
+}
+
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF = 0, int NT = 0>
 int launch_bk13t(size_t nelmt, const double* B, const double* D, const double* Q, const double* u, double* v,
                  hipStream_t s)
 {
@@ -66,7 +140,7 @@ int launch_bk13t(size_t nelmt, const double* B, const double* D, const double* Q
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk13t_kernel<NM, NQ, BK3, DIV, SB, NB, GPF>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, Q, u, v);
+    hipLaunchKernelGGL((bk13t_kernel<NM, NQ, BK3, DIV, SB, NB, GPF, NT>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, Q, u, v);
     return launch_status();
 }
 
@@ -78,7 +152,15 @@ int launch_bk13t_variant(int div_variant, size_t nelmt, const double* B, const d
     // scheduling barrier per row (three LDS boxes each); 4 all, 5 half (two
     // LDS boxes); BK3 only: 6 all, 7 half (two boxes, three rows of G loaded
     // with u), 8 all (two boxes, two rows); BK1 6, 7, 8 are 4, 5, 4.
-    // Variant 3 is the LDS-contraction mapping of bk13_f64.hip.
+    // Variant 3 is the LDS-contraction mapping of bk13_f64.hip.  9-12 are 6,
+    // 8, 7 and 4 with non-temporal loads of u and JxW/G and stores of v
+    // (each G plane is read by one instruction, so the streaming hint loses
+    // no cache reuse).  BK3 13-15: non-temporal, min(6, nq) rows of G or all
+    // nq rows loaded with u (all elements or half of them per workgroup);
+    // BK1 13-15 are 12.  16-19 are 9-12 with G (JxW) read by ordinary loads
+    // and only u and v streamed.  BK3 20-23: the persistent pipelined kernel
+    // (three or two rows of G in the ring, all or half the elements per
+    // workgroup; 23 without an occupancy target); BK1 20-23 are 19.
     switch (div_variant) {
     case 0:
         return launch_bk13t<NM, NM + 1, BK3, 1, false, 3>(nelmt, B, D, Q, u, v, s);
@@ -96,6 +178,46 @@ int launch_bk13t_variant(int div_variant, size_t nelmt, const double* B, const d
         return launch_bk13t<NM, NM + 1, BK3, 2, false, 2, BK3 ? 3 : 0>(nelmt, B, D, Q, u, v, s);
     case 8:
         return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, BK3 ? 2 : 0>(nelmt, B, D, Q, u, v, s);
+    case 9:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, BK3 ? 3 : 0, 3>(nelmt, B, D, Q, u, v, s);
+    case 10:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, BK3 ? 2 : 0, 3>(nelmt, B, D, Q, u, v, s);
+    case 11:
+        return launch_bk13t<NM, NM + 1, BK3, 2, false, 2, BK3 ? 3 : 0, 3>(nelmt, B, D, Q, u, v, s);
+    case 12:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, 0, 3>(nelmt, B, D, Q, u, v, s);
+    case 13:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, BK3 ? (NM + 1 < 6 ? NM + 1 : 6) : 0, 3>(nelmt, B, D, Q, u, v, s);
+    case 14:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, BK3 ? NM + 1 : 0, 3>(nelmt, B, D, Q, u, v, s);
+    case 15:
+        return launch_bk13t<NM, NM + 1, BK3, 2, false, 2, BK3 ? NM + 1 : 0, 3>(nelmt, B, D, Q, u, v, s);
+    case 16:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, BK3 ? 3 : 0, 1>(nelmt, B, D, Q, u, v, s);
+    case 17:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, BK3 ? 2 : 0, 1>(nelmt, B, D, Q, u, v, s);
+    case 18:
+        return launch_bk13t<NM, NM + 1, BK3, 2, false, 2, BK3 ? 3 : 0, 1>(nelmt, B, D, Q, u, v, s);
+    case 19:
+        return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, 0, 1>(nelmt, B, D, Q, u, v, s);
+    case 20:
+    case 21:
+    case 22:
+    case 23:
+        if constexpr (BK3) {
+            switch (div_variant) {
+            case 20:
+                return launch_bk3p<NM, NM + 1, 1, 3, 1>(nelmt, B, D, Q, u, v, s);
+            case 21:
+                return launch_bk3p<NM, NM + 1, 1, 2, 1>(nelmt, B, D, Q, u, v, s);
+            case 22:
+                return launch_bk3p<NM, NM + 1, 2, 3, 1>(nelmt, B, D, Q, u, v, s);
+            default:
+                return launch_bk3p<NM, NM + 1, 1, 3, 1, 1>(nelmt, B, D, Q, u, v, s);
+            }
+        } else {
+            return launch_bk13t<NM, NM + 1, BK3, 1, false, 2, 0, 1>(nelmt, B, D, Q, u, v, s);
+        }
     default:
         return BKX_ERROR_ARG;
     }

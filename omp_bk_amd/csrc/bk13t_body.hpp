@@ -66,9 +66,13 @@ struct Form<BKX_FORM_FF> {
     using L = ff;
 };
 
-__device__ __forceinline__ double load_as(const double* p, double*) { return *p; }
-__device__ __forceinline__ float load_as(const float* p, float*) { return *p; }
-__device__ __forceinline__ ff load_as(const double* p, ff*) { return ff_split(*p); }
+// loads of the streamed inputs (u, JxW, G), non-temporal when NT is set
+template <bool NT>
+__device__ __forceinline__ double load_as(const double* p, double*) { return load<NT>(p); }
+template <bool NT>
+__device__ __forceinline__ float load_as(const float* p, float*) { return load<NT>(p); }
+template <bool NT>
+__device__ __forceinline__ ff load_as(const double* p, ff*) { return ff_split(load<NT>(p)); }
 
 // Sum of products, in ascending order of the terms.  FP64: a running FMA
 // chain (the first product rounded alone: -0.0 + x is x for every x);
@@ -114,8 +118,20 @@ __device__ __forceinline__ double to_v(float x, double*) { return double(x); }
 __device__ __forceinline__ ff to_v(float x, ff*) { return ff{x, 0.0f}; }
 __device__ __forceinline__ ff to_v(ff x, ff*) { return x; }
 
-__device__ __forceinline__ void store_v(double* p, double x) { *p = x; }
-__device__ __forceinline__ void store_v(double* p, ff x) { *p = ff_join(x); }
+template <bool NT>
+__device__ __forceinline__ void store_v(double* p, double x)
+{
+    if constexpr (NT) {
+        st_stream(p, x);
+    } else {
+        *p = x;
+    }
+}
+template <bool NT>
+__device__ __forceinline__ void store_v(double* p, ff x)
+{
+    store_v<NT>(p, ff_join(x));
+}
 
 // the rows of B or D: FP64 through a scalar-load pointer, or the LDS copy
 template <int TAG>
@@ -235,6 +251,10 @@ __device__ __forceinline__ void split_bases(const double* __restrict__ B, const 
     }
 }
 
+// Values of JxW / G a lane holds ahead of use: BK1 all nq of its JxW(e, :, lane),
+// BK3 GPF rows of six factors (one when GPF = 0).
+constexpr int bk13t_nqv(bool bk3, int gpf, int nq) { return bk3 ? (gpf > 0 ? gpf * 6 : 1) : nq; }
+
 // One workgroup's pass over up to epb elements.  ue, Qe and ve are this
 // lane's element's u, JxW/G and v.  When !active they must still point at a
 // whole readable element of the same order: the loads are unconditional
@@ -257,13 +277,82 @@ __device__ __forceinline__ void split_bases(const double* __restrict__ B, const 
 //        Wr and Ws after a barrier.
 // GPF > 0 (BK3): P5 keeps GPF rows of G in flight (row p + GPF is loaded
 // when row p is used).  With GPF = 0, P5 loads one row ahead.
-template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB = 3, int GPF = 0, int FORM = BKX_FORM_F64,
-          typename MT = double>
-__device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* __restrict__ D,
-                                           const typename Form<FORM>::TS* __restrict__ Qe,
-                                           const typename Form<FORM>::TS* __restrict__ ue,
-                                           double* __restrict__ ve, bool active, int slot, int l, double* smem,
-                                           int dep = 0)
+// NT: bit 0 makes the loads of u and the stores of v non-temporal, bit 1
+// the loads of JxW / G.  A G plane G(e, f, p, :, :) is nq^2 values, which
+// for most orders begins and ends inside a 128-byte line shared with the
+// next plane; the next row reads that line again a little later, so
+// streaming G can cost a second fetch of it.
+//
+// The body is bk13t_load (the loads above, into xin and qin) followed by
+// bk13t_phases.  With PIPE (BK3, GPF > 0: the persistent kernel of
+// bk13t.hip) the phases also load the workgroup's next element: the last
+// GPF refills of the G ring in P5 fetch rows 0 .. GPF - 1 of the next
+// element (Qn), and after P5 the lane's next u column (un) replaces xin, so
+// that the next pass starts with its first loads already in flight.  In that
+// mode the loads of u are unconditional (lanes outside the element read a
+// valid column, clamped) and the prologue loads G before u, as the phases
+// do, so that the memory operations in flight at the top of the loop are the
+// same on both paths into it and the compiler's wait counts stay exact.
+template <int NM, int NQ, bool BK3, int DIV, int GPF, int FORM, int NT, bool PIPE>
+__device__ __forceinline__ void bk13t_load(const typename Form<FORM>::TS* __restrict__ Qe,
+                                           const typename Form<FORM>::TS* __restrict__ ue, int slot, int l,
+                                           typename Form<FORM>::L (&xin)[NM],
+                                           typename Form<FORM>::L (&qin)[bk13t_nqv(BK3, GPF, NQ + 0)])
+{
+    using L = typename Form<FORM>::L;
+    using S = Bk13tShape<NM, NQ, DIV>;
+    constexpr int nq2 = S::nq2;
+    const bool in_elem = slot < S::epb;
+    L* const tagL = nullptr;   // selects load_as overloads
+    const auto* ge = Qe + l;   // BK3: G(e, f, p, q, r) at f = p = 0, (q, r) = lane
+    if constexpr (PIPE) {
+        static_assert(BK3 && GPF > 0, "the pipelined body streams BK3's G ring");
+#pragma unroll
+        for (int p = 0; p < GPF; ++p) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                qin[p * 6 + f] = load_as<(NT & 2) != 0>(ge + f * S::nq3 + p * nq2, tagL);
+            }
+        }
+        const int lu = l < S::nm2 ? l : 0;
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            xin[i] = load_as<(NT & 1) != 0>(ue + i * S::nm2 + lu, tagL);
+        }
+        return;
+    }
+    if (in_elem && l < S::nm2) {
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            xin[i] = load_as<(NT & 1) != 0>(ue + i * S::nm2 + l, tagL);
+        }
+    }
+    if (in_elem) {
+        if constexpr (!BK3) {
+#pragma unroll
+            for (int t = 0; t < NQ; ++t) {
+                qin[t] = load_as<(NT & 2) != 0>(Qe + t * nq2 + l, tagL);
+            }
+        } else if constexpr (GPF > 0) {
+#pragma unroll
+            for (int p = 0; p < GPF; ++p) {
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    qin[p * 6 + f] = load_as<(NT & 2) != 0>(ge + f * S::nq3 + p * nq2, tagL);
+                }
+            }
+        }
+    }
+}
+
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF, int FORM, typename MT, int NT, bool PIPE>
+__device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT* __restrict__ D,
+                                             const typename Form<FORM>::TS* __restrict__ Qe,
+                                             double* __restrict__ ve, bool active, int slot, int l, double* smem,
+                                             int dep, typename Form<FORM>::L (&xin)[NM],
+                                             typename Form<FORM>::L (&qin)[bk13t_nqv(BK3, GPF, NQ + 0)],
+                                             const typename Form<FORM>::TS* __restrict__ Qn,
+                                             const typename Form<FORM>::TS* __restrict__ un)
 {
     static_assert(NB == 2 || NB == 3, "two or three LDS boxes per element");
     static_assert(GPF >= 0 && GPF <= NQ, "G prefetch depth");
@@ -288,33 +377,7 @@ __device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* _
     L* const tagL = nullptr;   // selects load_as overloads
     V* const tagV = nullptr;
 
-    // ---- loads ------------------------------------------------------------
-    L xin[NM];
-    constexpr int nqv = BK3 ? (GPF > 0 ? GPF * 6 : 1) : NQ;
-    L qin[nqv];
-    if (in_elem && l < S::nm2) {
-#pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            xin[i] = load_as(ue + i * S::nm2 + l, tagL);
-        }
-    }
     const auto* ge = Qe + l;   // BK3: G(e, f, p, q, r) at f = p = 0, (q, r) = lane
-    if (in_elem) {
-        if constexpr (!BK3) {
-#pragma unroll
-            for (int t = 0; t < NQ; ++t) {
-                qin[t] = load_as(Qe + t * nq2 + l, tagL);
-            }
-        } else if constexpr (GPF > 0) {
-#pragma unroll
-            for (int p = 0; p < GPF; ++p) {
-#pragma unroll
-                for (int f = 0; f < 6; ++f) {
-                    qin[p * 6 + f] = load_as(ge + f * S::nq3 + p * nq2, tagL);
-                }
-            }
-        }
-    }
 
     // ---- P1: lane (j,k), contract i -> p ---------------------------------
     if (in_elem && l < S::nm2) {
@@ -453,7 +516,7 @@ __device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* _
             if constexpr (GPF == 0) {
 #pragma unroll
                 for (int f = 0; f < 6; ++f) {
-                    gn[f] = load_as(ge + f * S::nq3, tagL);
+                    gn[f] = load_as<(NT & 2) != 0>(ge + f * S::nq3, tagL);
                 }
             }
 #pragma unroll
@@ -467,7 +530,7 @@ __device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* _
                     if (p + 1 < NQ) {
 #pragma unroll
                         for (int f = 0; f < 6; ++f) {
-                            gn[f] = load_as(ge + f * S::nq3 + (p + 1) * nq2, tagL);
+                            gn[f] = load_as<(NT & 2) != 0>(ge + f * S::nq3 + (p + 1) * nq2, tagL);
                         }
                     }
                 } else {
@@ -478,7 +541,17 @@ __device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* _
                     if (p + GPF < NQ) {
 #pragma unroll
                         for (int f = 0; f < 6; ++f) {
-                            qin[(p % GPF) * 6 + f] = load_as(ge + f * S::nq3 + (p + GPF) * nq2, tagL);
+                            qin[(p % GPF) * 6 + f] = load_as<(NT & 2) != 0>(ge + f * S::nq3 + (p + GPF) * nq2, tagL);
+                        }
+                    } else if constexpr (PIPE) {
+                        // the next element's row p % GPF: the slot just freed
+                        // is the one that row is read from (rows 0 .. GPF - 1
+                        // of the next element each take their own slot over
+                        // the last GPF rows, whatever nq mod GPF is)
+#pragma unroll
+                        for (int f = 0; f < 6; ++f) {
+                            qin[(p % GPF) * 6 + f] =
+                                load_as<(NT & 2) != 0>(Qn + l + f * S::nq3 + (p % GPF) * nq2, tagL);
                         }
                     }
                 }
@@ -504,6 +577,14 @@ __device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* _
                     // in flight, not all of G hoisted to the top of P5
                     __builtin_amdgcn_sched_barrier(0);
                 }
+            }
+        }
+        if constexpr (PIPE) {
+            // the lane's u column of the next element
+            const int lu = l < S::nm2 ? l : 0;
+#pragma unroll
+            for (int i = 0; i < NM; ++i) {
+                xin[i] = load_as<(NT & 1) != 0>(un + i * S::nm2 + lu, tagL);
             }
         }
         __syncthreads();
@@ -600,9 +681,24 @@ __device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* _
         contract_rows<NQ, SB>(mat<12>(B, dep), z, o);
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            store_v(ve + i * S::nm2 + l, o[i]);
+            store_v<(NT & 1) != 0>(ve + i * S::nm2 + l, o[i]);
         }
     }
+}
+
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB = 3, int GPF = 0, int FORM = BKX_FORM_F64,
+          typename MT = double, int NT = 0>
+__device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* __restrict__ D,
+                                           const typename Form<FORM>::TS* __restrict__ Qe,
+                                           const typename Form<FORM>::TS* __restrict__ ue,
+                                           double* __restrict__ ve, bool active, int slot, int l, double* smem,
+                                           int dep = 0)
+{
+    typename Form<FORM>::L xin[NM];
+    typename Form<FORM>::L qin[bk13t_nqv(BK3, GPF, NQ)];
+    bk13t_load<NM, NQ, BK3, DIV, GPF, FORM, NT, false>(Qe, ue, slot, l, xin, qin);
+    bk13t_phases<NM, NQ, BK3, DIV, SB, NB, GPF, FORM, MT, NT, false>(B, D, Qe, ve, active, slot, l, smem, dep, xin, qin,
+                                                                     nullptr, nullptr);
 }
 
 }  // namespace bk13t

@@ -66,8 +66,8 @@ struct VariantTable {
         //                            p:   0   1   2   3   4   5   6   7   8
         static const int bk5_f64[kOrders] = {0, 4, 26, 20, 34, 38, 38, 36, 0};
         static const int bk5_f32w[kOrders] = {0, 28, 33, 15, 23, 39, 39, 40, 0};
-        static const int bk3_f64[kOrders] = {0, 1, 8, 6, 7, 6, 8, 6, 7};
-        static const int bk1_f64[kOrders] = {0, 1, 5, 5, 5, 4, 5, 4, 5};
+        static const int bk3_f64[kOrders] = {0, 5, 11, 7, 18, 18, 13, 16, 18};
+        static const int bk1_f64[kOrders] = {0, 2, 7, 5, 6, 8, 11, 9, 11};
         static const int bk3_f32w[kOrders] = {0, 1, 1, 2, 2, 0, 0, 2, 2};
         static const int bk3_f32c[kOrders] = {0, 2, 2, 2, 0, 2, 1, 0, 0};
         static const int bk3_ff[kOrders] = {0, 2, 2, 2, 2, 2, 0, 0, 1};
@@ -139,8 +139,11 @@ int bkx_tune_count(int op, int form)
         // 1 half, 2 all with a scheduling barrier per row (three LDS boxes
         // per element), 4 all, 5 half (two LDS boxes), 6 all, 7 half, 8 all
         // (two boxes, BK3 with three, three and two rows of G loaded ahead;
-        // BK1: as 4, 5, 4); 3: LDS-contraction mapping (bk13_f64.hip)
-        return 9;
+        // BK1: as 4, 5, 4); 3: LDS-contraction mapping (bk13_f64.hip);
+        // 9-12: 6, 8, 7, 4 with non-temporal streams; 13-15: more rows of
+        // G loaded ahead; 16-19: 9-12 with G read by ordinary loads;
+        // 20-23: persistent pipelined BK3
+        return 24;
     }
     return (op == 1 || op == 3 || op == 5) && form >= 0 && form < kForms ? 1 : -1;
 }

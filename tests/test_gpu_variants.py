@@ -73,3 +73,70 @@ def test_variant(op, form, variant, p):
     finally:
         _lib.call("bkx_tune_set", op, f, p, prev)
         assert L.bkx_tune_get(op, f, p) == prev
+
+
+# The persistent launch variants (each workgroup walks a strided sequence of
+# element groups, prefetching the next group's G and u) only loop when there
+# are more element groups than resident workgroups, which the ragged cases
+# above never reach.  These cases launch 8192 groups (several passes per
+# workgroup on any MI355X partition) with device-generated data, compare the
+# whole output with variant 0 and sampled elements with the oracle.
+PERSISTENT = {(5, "f64"): (30, 31, 32, 33, 39, 40, 41), (5, "f32w"): (30, 31, 32, 33, 39, 40, 41),
+              (3, "f64"): (20, 21, 22, 23)}
+
+
+def _persistent_cases():
+    out = []
+    for (op, form), variants in PERSISTENT.items():
+        orders = range(5, 8) if op == 5 else range(4, 9)
+        for v in variants:
+            for p in orders:
+                out.append((op, form, v, p))
+    return out
+
+
+@pytest.mark.parametrize("op,form,variant,p", _persistent_cases())
+def test_persistent_variant_loops(op, form, variant, p):
+    L = _lib.lib()
+    f = FORM[form]
+    dt = torch.float64 if form == "f64" else torch.float32
+    gen = torch.Generator(device=DEV).manual_seed(1000 * variant + p)
+    if op == 5:
+        n = p + 1
+        E = 8192 * max(1, 64 // n ** 2)
+        D = O.test_basis(n, n)
+        G = (torch.rand(E * 6 * n ** 3, generator=gen, device=DEV, dtype=torch.float64) + 1.0).to(dt)
+        u = (torch.rand(E * n ** 3, generator=gen, device=DEV, dtype=torch.float64) * 2 - 1).to(dt)
+        run = lambda out: ops.bk5(p, _dev(D), G, u, out=out, form=form)
+        per_u, per_g = n ** 3, 6 * n ** 3
+        ref_of = lambda g, x: O.bk5(n, D, g, x)
+    else:
+        nm, nq = p + 1, p + 2
+        E = 8192 * max(1, 256 // nq ** 2)
+        B, Dm = O.test_basis(nm, nq), O.test_basis(nq, nq)
+        G = torch.rand(E * 6 * nq ** 3, generator=gen, device=DEV, dtype=torch.float64) + 1.0
+        u = torch.rand(E * nm ** 3, generator=gen, device=DEV, dtype=torch.float64) * 2 - 1
+        run = lambda out: ops.bk3(p, _dev(B), _dev(Dm), G, u, out=out, form=form)
+        per_u, per_g = nm ** 3, 6 * nq ** 3
+        ref_of = lambda g, x: O.bk3(nq, B, Dm, g, x)
+    prev = L.bkx_tune_get(op, f, p)
+    try:
+        base = torch.full((u.numel(),), float("nan"), dtype=torch.float64, device=DEV)
+        got = torch.full_like(base, float("nan"))
+        _lib.call("bkx_tune_set", op, f, p, 0)
+        run(base)
+        _lib.call("bkx_tune_set", op, f, p, variant)
+        run(got)
+        torch.cuda.synchronize()
+        assert bool(torch.isfinite(got).all())
+        err = float(((got - base).abs().max() / base.abs().max()).item())
+        assert err <= BAR[form], err
+        # oracle on the first, a middle and the last elements
+        for e0 in (0, E // 2 + 1, E - 5):
+            ne = 5
+            g = G[e0 * per_g:(e0 + ne) * per_g].double().cpu().numpy()
+            x = u[e0 * per_u:(e0 + ne) * per_u].double().cpu().numpy()
+            ref = ref_of(g, x)
+            assert O.rel_maxdiff(got[e0 * per_u:(e0 + ne) * per_u].cpu().numpy(), ref) <= BAR[form]
+    finally:
+        _lib.call("bkx_tune_set", op, f, p, prev)
