@@ -128,7 +128,11 @@ def reduced_forms(w, steps, warmup, ms64):
     G32, u32 = w["G"].float(), w["u"].float()
     ref_q = ops.bk5(p, w["D"], G32.double(), u32.double())
     out = {}
-    for form in ("f32c", "f32w", "bf16s"):
+    # the FP32-storage form first, right after the FP64 line it is compared
+    # with; the VALU-bound float-float form last (on MI355X a long
+    # arithmetic-bound run lowers the clocks of the memory-bound runs that
+    # follow it: timed after f32c, f32w loses about 15%)
+    for form in ("f32w", "bf16s", "f32c"):
         v = torch.empty_like(w["v"], dtype=torch.float32 if form == "bf16s" else torch.float64)
         nb = 32 if form == "bf16s" else 36
         fn = lambda: ops.bk5(p, w["D"], G32, u32, out=v, form=form)

@@ -23,7 +23,10 @@ struct Bk13tShape {
     static constexpr int full = 256 / nq2 > 0 ? 256 / nq2 : 1;
     static constexpr int epb = full / DIV > 0 ? full / DIV : 1;
     static constexpr int threads = ((epb * nq2 + kWave - 1) / kWave) * kWave;
-    static constexpr int row = NQ + 1;
+    // an odd row stride (in 8-byte words) spreads every lane plane's column
+    // accesses over the LDS banks: NQ + 1 when NQ is even, NQ when it is odd
+    // (NQ + 1 even doubles the bank conflicts of the transposes at odd NQ)
+    static constexpr int row = NQ | 1;
     static constexpr int box = NQ * NQ * row;
 };
 
@@ -395,7 +398,7 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
 #pragma unroll
             for (int t = 0; t < NQ; ++t) {
                 const int x = t * nq2 + l;
-                bC[x + x / NQ] = to_v(qin[t], tagV);
+                bC[x + (x / NQ) * (R - NQ)] = to_v(qin[t], tagV);
             }
         }
     }
@@ -419,7 +422,7 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
 #pragma unroll
                 for (int t = 0; t < NQ; ++t) {
                     const int x = t * nq2 + l;
-                    bA[x + x / NQ] = to_v(qin[t], tagV);
+                    bA[x + (x / NQ) * (R - NQ)] = to_v(qin[t], tagV);
                 }
             }
         }

@@ -106,7 +106,7 @@ struct Elem {
     static constexpr int nm3 = nm2 * NM;
     static constexpr int nq2 = NQ * NQ;
     static constexpr int nq3 = nq2 * NQ;
-    static constexpr int row = NQ + 1;
+    static constexpr int row = NQ | 1;            // odd row stride in 8-byte words (bank spread)
     static constexpr int box = NQ * NQ * row;
 
     double* A;

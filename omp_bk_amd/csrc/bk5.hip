@@ -97,7 +97,7 @@ struct Bk5Shape {
     static constexpr int nq2 = NQ * NQ;
     static constexpr int nq3 = nq2 * NQ;
     static constexpr int epw = kWave / nq2;       // elements per wavefront
-    static constexpr int row = NQ + 1;            // padded LDS row (bank spread)
+    static constexpr int row = NQ | 1;            // odd row stride in 8-byte words (bank spread)
     static constexpr int plane = NQ * row;        // one padded (j, k) plane
     static constexpr int box = NQ * plane;        // one padded element box
     static_assert(epw >= 1, "one element must fit a wavefront");

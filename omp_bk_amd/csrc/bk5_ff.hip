@@ -29,7 +29,7 @@ struct Bk5FfShape {
     static constexpr int nq2 = NQ * NQ;
     static constexpr int nq3 = nq2 * NQ;
     static constexpr int epw = kWave / nq2;
-    static constexpr int row = NQ + 1;
+    static constexpr int row = NQ | 1;            // odd row stride in 8-byte words (bank spread)
     static constexpr int plane = NQ * row;
     static constexpr int box = NQ * plane;
 };
