@@ -4,9 +4,13 @@ A step is one apply of the operator over the whole workload (inputs resident
 in HBM).  The headline workload is BK5 (collocated Laplacian) at p = 7:
 n = q = 8 GLL points per direction, E = 65536 elements of a 64 x 32 x 32
 perturbed box mesh (33.5 M DoF), six geometric factors per point, in FP64;
-u = uniform[-1, 1] + sin(pi x) sin(pi y) sin(pi z).  BK3 at p = 7 is timed
-beside it.  Rates are GDoF/s = E n^3 / t; algorithmic bytes are in + out +
-geometric factors (openmp/BK5.cpp:143-144, openmp/BK3.cpp:50-51).
+u = uniform[-1, 1] + sin(pi x) sin(pi y) sin(pi z).  Timed beside it: the
+FP32-storage and split-BF16 forms of the same workload, BK3 at p = 7 on the
+same mesh and at p = 5 (E = 32768), BK1 at p = 3 (E = 4096), the BK3
+adjoint (p = 6, E = 16384), the mixed-order BK3 batch (200,000 elements),
+the float-float form, and the reference's own CPU operator on a sample.
+Rates are GDoF/s = E n^3 / t; algorithmic bytes are in + out + geometric
+factors (openmp/BK5.cpp:143-144, openmp/BK3.cpp:50-51).
 
 Prints one JSON line.  python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -71,7 +75,9 @@ def bk5_workload(p=7, E=65536):
     return dict(fn=fn, dofs=E * n ** 3, bytes=bytes_per_dof * E * n ** 3, D=D, G=G, u=u, v=v, p=p, E=E)
 
 
-def bk3_workload(p=7, E=16384):
+def bk3_workload(p=7, E=65536):
+    """BK3 FP64 on the perturbed box mesh (64 x 32 x 32 elements at
+    E = 65536, 32 x 32 x 32 at E = 32768), u = U[-1, 1]."""
     import torch
     from omp_bk_amd import geometry as geo
     from omp_bk_amd import ops
@@ -84,6 +90,74 @@ def bk3_workload(p=7, E=16384):
     v = torch.empty_like(u)
     fn = lambda: ops.bk3(p, B, D, G, u, out=v)
     return dict(fn=fn, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + 6 * E * q ** 3), p=p, E=E)
+
+
+def bk1_workload(p=3, E=4096):
+    """BK1 FP64, p = 3 on the 16 x 16 x 16 perturbed box mesh (the reference's
+    CPU-run configuration), u = U[-1, 1]."""
+    import torch
+    from omp_bk_amd import geometry as geo
+    from omp_bk_amd import ops
+    dev = "cuda:0"
+    n, q = p + 1, p + 2
+    torch.manual_seed(555)
+    B, _ = (torch.as_tensor(a, device=dev) for a in geo.bk13_bases(p))
+    J = geo.geometric_factors(1, p, E, device=dev).reshape(-1)
+    u = torch.rand(E * n ** 3, dtype=torch.float64, device=dev) * 2 - 1
+    v = torch.empty_like(u)
+    fn = lambda: ops.bk1(p, B, J, u, out=v)
+    return dict(fn=fn, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + E * q ** 3), p=p, E=E)
+
+
+def rate(w, ms):
+    """GDoF/s, algorithmic GB/s and fraction of the 8 TB/s peak."""
+    gbs = w["bytes"] / (ms * 1e-3) * 1e-9
+    return {"value": round(w["dofs"] / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "nelmt": w["E"],
+            "ms_per_step": round(ms, 5), "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bytes_per_dof": round(w["bytes"] / w["dofs"], 3)}
+
+
+def adjoint_config(steps, warmup, p=6, E=16384):
+    """The BK3 adjoint configuration: L = 1/2 |A(kappa) u - b|^2 with kappa
+    log-normal (sigma = 0.5), u, b = U[-1, 1], p = 6, E = 16384; one step is
+    the fused adjoint (dL/du, dL/dkappa and the per-element loss) plus the
+    fixed-order sum of L.  FP64, and FP32 storage of u, b, kappa and G with
+    FP64 arithmetic (errors against FP64 on the same rounded inputs).  Bytes
+    per DoF count every input once and both outputs (G and kappa are read a
+    second time from the L2 cache)."""
+    import torch
+    from omp_bk_amd import geometry as geo
+    from omp_bk_amd import ops
+    dev = "cuda:0"
+    n, q = p + 1, p + 2
+    g = torch.Generator(device=dev).manual_seed(2024)
+    B, D = (torch.as_tensor(a, device=dev) for a in geo.bk13_bases(p))
+    G = geo.geometric_factors(3, p, E, device=dev).reshape(-1)
+    kappa = torch.exp(0.5 * torch.randn(E * q ** 3, dtype=torch.float64, device=dev, generator=g))
+    u = torch.rand(E * n ** 3, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    b = torch.rand(E * n ** 3, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    dofs = E * n ** 3
+    out = {}
+    ref = None
+    for form, s in (("f64", 8), ("f32w", 4)):
+        cast = (lambda x: x.float()) if form == "f32w" else (lambda x: x)
+        Gs, ks, us, bs = cast(G), cast(kappa), cast(u), cast(b)
+        if form == "f32w":
+            ref = ops.bk3_adjoint(p, B, D, Gs.double(), ks.double(), us.double(), bs.double())
+        res = {}
+        fn = lambda: res.__setitem__("r", ops.bk3_adjoint(p, B, D, Gs, ks, us, bs, form=form))
+        ms, _ = time_steps(fn, steps, warmup)
+        nbytes = s * (2 * dofs + 7 * E * q ** 3) + 8 * (dofs + E * q ** 3)
+        line = {"value": round(dofs / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
+                "bytes_per_dof": round(nbytes / dofs, 2), "achieved_GBs": round(nbytes / (ms * 1e-3) * 1e-9, 1)}
+        if ref is not None:
+            L, du, dk, _ = res["r"]
+            rel = lambda x, r: float(((x - r).abs().max() / r.abs().max()).item())
+            line["relerr_vs_f64_same_inputs"] = {"L": abs(L.item() - ref[0].item()) / ref[0].item(),
+                                                 "dLdu": rel(du, ref[1]), "dLdkappa": rel(dk, ref[2])}
+        out["bk3_adjoint_p6_" + form] = line
+        del Gs, ks, us, bs
+    return out
 
 
 def mixed_batch(steps, warmup, nelmt=200000):
@@ -117,7 +191,7 @@ def mixed_batch(steps, warmup, nelmt=200000):
     return out
 
 
-def reduced_forms(w, steps, warmup, ms64):
+def reduced_forms(w, steps, warmup, ms64, forms=("f32w", "bf16s", "f32c")):
     """BK5 p=7 with u and G stored in FP32 (v in FP64): the float-float
     compensated form f32c and the FP64-arithmetic form f32w.  Errors are
     max|v - v64| / max|v64| against the FP64 operator on the same FP32-rounded
@@ -128,11 +202,7 @@ def reduced_forms(w, steps, warmup, ms64):
     G32, u32 = w["G"].float(), w["u"].float()
     ref_q = ops.bk5(p, w["D"], G32.double(), u32.double())
     out = {}
-    # the FP32-storage form first, right after the FP64 line it is compared
-    # with; the VALU-bound float-float form last (on MI355X a long
-    # arithmetic-bound run lowers the clocks of the memory-bound runs that
-    # follow it: timed after f32c, f32w loses about 15%)
-    for form in ("f32w", "bf16s", "f32c"):
+    for form in forms:
         v = torch.empty_like(w["v"], dtype=torch.float32 if form == "bf16s" else torch.float64)
         nb = 32 if form == "bf16s" else 36
         fn = lambda: ops.bk5(p, w["D"], G32, u32, out=v, form=form)
@@ -231,7 +301,7 @@ def run_only(a):
                                torch.cuda.current_stream().cuda_stream)
         nbytes = 2 * 8 * n
     elif a.only == "bk3":
-        w = bk3_workload()
+        w = bk3_workload(7, 65536)
         fn, nbytes = w["fn"], w["bytes"]
     else:
         w = bk5_workload()
@@ -266,15 +336,25 @@ def main():
     gdofs5 = w5["dofs"] / (ms5 * 1e-3) * 1e-9
     gbs5 = w5["bytes"] / (ms5 * 1e-3) * 1e-9
 
-    forms = reduced_forms(w5, a.steps, a.warmup, ms5)
+    # The FP32-storage forms right after the FP64 line they are compared
+    # with, the memory-bound lines next, and the arithmetic-bound
+    # float-float form last: timed after it, memory-bound kernels ran up to
+    # 15% slower on MI355X.
+    forms = reduced_forms(w5, a.steps, a.warmup, ms5, forms=("f32w", "bf16s"))
+
+    lines = {}
+    for name, make in (("bk3_p7_f64", lambda: bk3_workload(7, 65536)), ("bk3_p5_f64", lambda: bk3_workload(5, 32768)),
+                       ("bk1_p3_f64", lambda: bk1_workload(3, 4096))):
+        w = make()
+        ms, _ = time_steps(w["fn"], a.steps, a.warmup)
+        lines[name] = rate(w, ms)
+        del w
+    lines.update(adjoint_config(a.steps, a.warmup))
+    torch.cuda.empty_cache()
 
     mix = mixed_batch(a.steps, a.warmup)
 
-    w3 = bk3_workload()
-    ms3, _ = time_steps(w3["fn"], a.steps, a.warmup)
-    gdofs3 = w3["dofs"] / (ms3 * 1e-3) * 1e-9
-    gbs3 = w3["bytes"] / (ms3 * 1e-3) * 1e-9
-    del w3
+    forms.update(reduced_forms(w5, a.steps, a.warmup, ms5, forms=("f32c",)))
 
     cpu = None if a.no_cpu else cpu_baseline(w5)
 
@@ -300,10 +380,8 @@ def main():
         "frac_of_read7": round(gbs5 / roofs["read7"], 4),
         "wall_ms_per_step": round(wall5, 5),
         **forms,
+        **lines,
         "bk3_mixed": mix,
-        "bk3_p7_f64": {"value": round(gdofs3, 3), "unit": "GDoF/s", "nelmt": 16384, "ms_per_step": round(ms3, 5),
-                       "achieved_GBs": round(gbs3, 1), "frac": round(gbs3 / HBM_PEAK_GBS, 4),
-                       "bytes_per_dof": round(8 * (2 + 6 * (9 / 8) ** 3), 3)},
     }
     print(json.dumps(line), flush=True)
 

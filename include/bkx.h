@@ -181,6 +181,18 @@ int bkx_bk3_vjp_f64(int p, size_t nelmt, const double* basis, const double* dbas
                     const double* kappa, const double* u, const double* w, double* grad_u, double* grad_kappa,
                     bkx_stream_t stream);
 
+/* The same three with u, b (or w), kappa and G stored in FP32 and FP64
+ * arithmetic: the FP64 operator applied exactly to FP32 inputs (half the
+ * input bytes); outputs and loss in FP64. */
+int bkx_bk3_apply_kappa_f32w(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                             const float* kappa, const float* in, double* out, bkx_stream_t stream);
+int bkx_bk3_adjoint_f32w(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                         const float* kappa, const float* u, const float* b, double* dldu, double* dldkappa,
+                         double* loss, bkx_stream_t stream);
+int bkx_bk3_vjp_f32w(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                     const float* kappa, const float* u, const float* w, double* grad_u, double* grad_kappa,
+                     bkx_stream_t stream);
+
 /* *out = sum of x[0..n-1] in a fixed order (one workgroup; deterministic). */
 int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream);
 

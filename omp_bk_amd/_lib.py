@@ -30,6 +30,9 @@ _SIGNATURES = {
     "bkx_bk3_apply_kappa_f64": [_c_int, _c_size] + [_vp] * 7,
     "bkx_bk3_adjoint_f64": [_c_int, _c_size] + [_vp] * 10,
     "bkx_bk3_vjp_f64": [_c_int, _c_size] + [_vp] * 9,
+    "bkx_bk3_apply_kappa_f32w": [_c_int, _c_size] + [_vp] * 7,
+    "bkx_bk3_adjoint_f32w": [_c_int, _c_size] + [_vp] * 10,
+    "bkx_bk3_vjp_f32w": [_c_int, _c_size] + [_vp] * 9,
     "bkx_sum_f64": [_vp, _c_size, _vp, _vp],
     "bkx_probe_copy": [_vp, _vp, _c_size, _vp],
     "bkx_probe_read_mix": [_c_int, _vp, _vp, _c_size, _vp],

@@ -18,6 +18,12 @@
 // are read twice, the second time while the element's first read is still
 // in the L2 cache.  The phases are those of the transposing BK3 mapping
 // (bk3_phases.hpp).
+//
+// TS is the storage type of the inputs u, b (or w), kappa and G: double for
+// the FP64 entry points, float for the FP32-storage forms (_f32w), which
+// convert each value to FP64 when it is loaded and compute in FP64, so that
+// they apply the FP64 operator to the FP32 inputs exactly (dL/du, dL/dkappa
+// and the loss are FP64 in both).
 #include "bk3_phases.hpp"
 #include "bkx_launch.hpp"
 
@@ -33,10 +39,10 @@ struct AdjShape {
     static constexpr int threads = ((epb * nq2 + kWave - 1) / kWave) * kWave;
 };
 
-template <int NM, int NQ, int MODE>
+template <int NM, int NQ, int MODE, typename TS>
 __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
-    size_t nelmt, const double* __restrict__ Bm, const double* __restrict__ Dm, const double* __restrict__ G,
-    const double* __restrict__ kappa, const double* __restrict__ u, const double* __restrict__ bw,
+    size_t nelmt, const double* __restrict__ Bm, const double* __restrict__ Dm, const TS* __restrict__ G,
+    const TS* __restrict__ kappa, const TS* __restrict__ u, const TS* __restrict__ bw,
     double* __restrict__ out_u, double* __restrict__ out_k, double* __restrict__ loss)
 {
     using S = AdjShape<NM, NQ>;
@@ -54,28 +60,49 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
     el.init(s_box[in_elem ? slot : 0], l, in_elem);
     const bool modal = in_elem && l < E::nm2;   // lane (j,k) of the modal plane
 
+    // Loads are unconditional, from clamped addresses inside a real element
+    // (element 0 for idle lanes, column 0 for lanes outside the modal plane),
+    // and masked after the conversion: a load whose converted value is
+    // needed when a conditional block ends is waited for there, which for
+    // FP32 storage serialised every load of the kernel.
+    const int lm = modal ? l : 0;
+    auto modal_load = [&](const TS* base, double (&dst)[NM]) {
+        TS raw[NM];
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            raw[i] = base[ee * E::nm3 + i * E::nm2 + lm];
+        }
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            dst[i] = (active && modal) ? double(raw[i]) : 0.0;
+        }
+    };
+
     // gradient of u
     double x[NM];
-#pragma unroll
-    for (int i = 0; i < NM; ++i) {
-        x[i] = (active && modal) ? u[ee * E::nm3 + i * E::nm2 + l] : 0.0;
-    }
+    modal_load(u, x);
     double qr[NQ];
     el.template grad<0>(Bm, Dm, x, qr);
 
     // F = G P u (kept), kappa F -> rr (registers), rs -> B, rt -> C
-    const double* ge = G + ee * 6 * E::nq3 + l;      // G(e, f, p, q, r) at f = p = 0
-    const double* ke = kappa + ee * E::nq3 + l;      // kappa(e, p, q, r) at p = 0
+    const TS* ge = G + ee * 6 * E::nq3 + l;      // G(e, f, p, q, r) at f = p = 0
+    const TS* ke = kappa + ee * E::nq3 + l;      // kappa(e, p, q, r) at p = 0
     double F0[NQ], F1[NQ], F2[NQ], rr[NQ];
     if (in_elem) {
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {
+            TS graw[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                graw[f] = ge[f * E::nq3 + p * nq2];
+            }
+            const TS kraw = ke[p * nq2];
             double g[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                g[f] = active ? ge[f * E::nq3 + p * nq2] : 0.0;
+                g[f] = active ? double(graw[f]) : 0.0;
             }
-            const double kap = active ? ke[p * nq2] : 0.0;
+            const double kap = active ? double(kraw) : 0.0;
             const int xx = E::X(p, el.lq, el.lq2);
             const double qs = el.B[xx];
             const double qt = el.C[xx];
@@ -91,10 +118,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
 
     double r[NM];
     if constexpr (MODE == kVjp) {
-#pragma unroll
-        for (int i = 0; i < NM; ++i) {
-            r[i] = (active && modal) ? bw[ee * E::nm3 + i * E::nm2 + l] : 0.0;
-        }
+        modal_load(bw, r);
     } else {
         double o[NM];
         el.template div<10>(Bm, Dm, rr, o);
@@ -107,10 +131,12 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
             }
             return;
         } else {
+            double bv[NM];
+            modal_load(bw, bv);
             double part = 0.0;
 #pragma unroll
             for (int i = 0; i < NM; ++i) {
-                r[i] = (active && modal) ? o[i] - bw[ee * E::nm3 + i * E::nm2 + l] : 0.0;
+                r[i] = (active && modal) ? o[i] - bv[i] : 0.0;
                 part += r[i] * r[i];
             }
             // 1/2 |r_e|^2: the element's lanes summed in lane order through box C
@@ -136,12 +162,18 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
     if (in_elem) {
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {
+            TS graw[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                graw[f] = ge[f * E::nq3 + p * nq2];
+            }
+            const TS kraw = ke[p * nq2];
             double g[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                g[f] = active ? ge[f * E::nq3 + p * nq2] : 0.0;
+                g[f] = active ? double(graw[f]) : 0.0;
             }
-            const double kap = active ? ke[p * nq2] : 0.0;
+            const double kap = active ? double(kraw) : 0.0;
             const int xx = E::X(p, el.lq, el.lq2);
             const double qs = el.B[xx];
             const double qt = el.C[xx];
@@ -164,16 +196,16 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
     }
 }
 
-template <int NM, int MODE>
-int launch_adj(size_t nelmt, const double* B, const double* D, const double* G, const double* kappa, const double* u,
-               const double* bw, double* out_u, double* out_k, double* loss, hipStream_t s)
+template <int NM, int MODE, typename TS>
+int launch_adj(size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
+               const TS* bw, double* out_u, double* out_k, double* loss, hipStream_t s)
 {
     using S = AdjShape<NM, NM + 1>;
     const unsigned grid = grid_for(nelmt, S::epb);
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk3_adj_kernel<NM, NM + 1, MODE>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, G, kappa,
+    hipLaunchKernelGGL((bk3_adj_kernel<NM, NM + 1, MODE, TS>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, G, kappa,
                        u, bw, out_u, out_k, loss);
     return launch_status();
 }
@@ -204,28 +236,68 @@ __global__ __launch_bounds__(kSumThreads) void sum_kernel(const double* __restri
 
 }  // namespace
 
+template <typename TS>
+int bk3_apply_kappa(int p, size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
+                    double* v, hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto P) {
+        return launch_adj<decltype(P)::value + 1, kApply, TS>(nelmt, B, D, G, kappa, u, nullptr, v, nullptr, nullptr,
+                                                              s);
+    });
+}
+
+template <typename TS>
+int bk3_adjoint(int p, size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
+                const TS* b, double* dldu, double* dldk, double* loss, hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto P) {
+        return launch_adj<decltype(P)::value + 1, kLoss, TS>(nelmt, B, D, G, kappa, u, b, dldu, dldk, loss, s);
+    });
+}
+
+template <typename TS>
+int bk3_vjp(int p, size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
+            const TS* w, double* gu, double* gk, hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto P) {
+        return launch_adj<decltype(P)::value + 1, kVjp, TS>(nelmt, B, D, G, kappa, u, w, gu, gk, nullptr, s);
+    });
+}
+
 int bk3_apply_kappa_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
                         const double* u, double* v, hipStream_t s)
 {
-    return with_order<1, 8>(p, [&](auto P) {
-        return launch_adj<decltype(P)::value + 1, kApply>(nelmt, B, D, G, kappa, u, nullptr, v, nullptr, nullptr, s);
-    });
+    return bk3_apply_kappa(p, nelmt, B, D, G, kappa, u, v, s);
 }
 
 int bk3_adjoint_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
                     const double* u, const double* b, double* dldu, double* dldk, double* loss, hipStream_t s)
 {
-    return with_order<1, 8>(p, [&](auto P) {
-        return launch_adj<decltype(P)::value + 1, kLoss>(nelmt, B, D, G, kappa, u, b, dldu, dldk, loss, s);
-    });
+    return bk3_adjoint(p, nelmt, B, D, G, kappa, u, b, dldu, dldk, loss, s);
 }
 
 int bk3_vjp_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
                 const double* u, const double* w, double* gu, double* gk, hipStream_t s)
 {
-    return with_order<1, 8>(p, [&](auto P) {
-        return launch_adj<decltype(P)::value + 1, kVjp>(nelmt, B, D, G, kappa, u, w, gu, gk, nullptr, s);
-    });
+    return bk3_vjp(p, nelmt, B, D, G, kappa, u, w, gu, gk, s);
+}
+
+int bk3_apply_kappa_f32w(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                         const float* u, double* v, hipStream_t s)
+{
+    return bk3_apply_kappa(p, nelmt, B, D, G, kappa, u, v, s);
+}
+
+int bk3_adjoint_f32w(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                     const float* u, const float* b, double* dldu, double* dldk, double* loss, hipStream_t s)
+{
+    return bk3_adjoint(p, nelmt, B, D, G, kappa, u, b, dldu, dldk, loss, s);
+}
+
+int bk3_vjp_f32w(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                 const float* u, const float* w, double* gu, double* gk, hipStream_t s)
+{
+    return bk3_vjp(p, nelmt, B, D, G, kappa, u, w, gu, gk, s);
 }
 
 int sum_f64(const double* x, size_t n, double* out, hipStream_t s)

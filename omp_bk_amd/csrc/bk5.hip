@@ -142,10 +142,17 @@ __global__ __launch_bounds__(kWave * WAVES) void bk5_kernel(size_t nelmt, const 
     // u column along i -> registers and the padded LDS box.
     const TS* ue = u + ee * nq3 + jk;
     double* su = s_u[wave][sl];
+    // unconditional loads (ee is a real element), masked after the
+    // conversion: converting inside the condition waits for each load
+    TS uraw[NQ];
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        uraw[i] = load<NT>(ue + i * nq2);
+    }
     double ur[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
-        ur[i] = active ? double(load<NT>(ue + i * nq2)) : 0.0;
+        ur[i] = active ? double(uraw[i]) : 0.0;
     }
     if (in_elem) {
 #pragma unroll
@@ -294,10 +301,17 @@ __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(M
 
     // u in L_i: lane (j, k) = (a, b), column i
     const TS* ue = u + ee * nq3 + a * NQ + b;
+    // unconditional loads (ee is a real element), masked after the
+    // conversion: converting inside the condition waits for each load
+    TS xraw[NQ];
+#pragma unroll
+    for (int c = 0; c < NQ; ++c) {
+        xraw[c] = load<NT>(ue + c * nq2);
+    }
     double x[NQ];
 #pragma unroll
     for (int c = 0; c < NQ; ++c) {
-        x[c] = active ? double(load<NT>(ue + c * nq2)) : 0.0;
+        x[c] = active ? double(xraw[c]) : 0.0;
     }
     // first layer of G
     const TS* ge = G + ee * 6 * nq3 + a * 6 * NQ + b;

@@ -22,6 +22,12 @@ int bk3_adjoint_f64(int p, size_t nelmt, const double* B, const double* D, const
                     const double* u, const double* b, double* dldu, double* dldk, double* loss, hipStream_t s);
 int bk3_vjp_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
                 const double* u, const double* w, double* gu, double* gk, hipStream_t s);
+int bk3_apply_kappa_f32w(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                         const float* u, double* v, hipStream_t s);
+int bk3_adjoint_f32w(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                     const float* u, const float* b, double* dldu, double* dldk, double* loss, hipStream_t s);
+int bk3_vjp_f32w(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                 const float* u, const float* w, double* gu, double* gk, hipStream_t s);
 int sum_f64(const double* x, size_t n, double* out, hipStream_t s);
 int bk5_apply_bf16s(int p, size_t nelmt, const double* D, const float* G, const float* u, float* v, hipStream_t s);
 int bk5_variants();
@@ -64,8 +70,8 @@ struct VariantTable {
             }
         }
         //                            p:   0   1   2   3   4   5   6   7   8
-        static const int bk5_f64[kOrders] = {0, 4, 26, 20, 34, 38, 38, 36, 0};
-        static const int bk5_f32w[kOrders] = {0, 28, 33, 15, 23, 39, 39, 40, 0};
+        static const int bk5_f64[kOrders] = {0, 4, 2, 1, 34, 38, 38, 36, 0};
+        static const int bk5_f32w[kOrders] = {0, 10, 16, 35, 27, 39, 22, 34, 0};
         static const int bk3_f64[kOrders] = {0, 5, 11, 7, 18, 18, 13, 16, 18};
         static const int bk1_f64[kOrders] = {0, 2, 7, 5, 6, 8, 11, 9, 11};
         static const int bk3_f32w[kOrders] = {0, 1, 1, 2, 2, 0, 0, 2, 2};
@@ -392,6 +398,53 @@ int bkx_bk3_vjp_f64(int p, size_t nelmt, const double* basis, const double* dbas
         return BKX_ERROR_NULL;
     }
     return bkx::bk3_vjp_f64(p, nelmt, basis, dbasis, G, kappa, u, w, grad_u, grad_kappa, as_stream(stream));
+}
+
+int bkx_bk3_apply_kappa_f32w(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                             const float* kappa, const float* in, double* out, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !kappa || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_apply_kappa_f32w(p, nelmt, basis, dbasis, G, kappa, in, out, as_stream(stream));
+}
+
+int bkx_bk3_adjoint_f32w(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                         const float* kappa, const float* u, const float* b, double* dldu, double* dldkappa,
+                         double* loss, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !kappa || !u || !b || !dldu || !dldkappa || !loss) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_adjoint_f32w(p, nelmt, basis, dbasis, G, kappa, u, b, dldu, dldkappa, loss, as_stream(stream));
+}
+
+int bkx_bk3_vjp_f32w(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                     const float* kappa, const float* u, const float* w, double* grad_u, double* grad_kappa,
+                     bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !kappa || !u || !w || !grad_u || !grad_kappa) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_vjp_f32w(p, nelmt, basis, dbasis, G, kappa, u, w, grad_u, grad_kappa, as_stream(stream));
 }
 
 int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream)

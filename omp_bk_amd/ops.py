@@ -148,59 +148,69 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
     return out
 
 
-def _bk3_common(p, basis, dbasis, G, u):
+_KAPPA_FORMS = {"f64": torch.float64, "f32w": torch.float32}
+
+
+def _bk3_common(p, basis, dbasis, G, u, form="f64"):
     _order_ok(3, p)
+    if form not in _KAPPA_FORMS:
+        raise ValueError("unknown BK3-kappa form %r (one of %s)" % (form, ", ".join(_KAPPA_FORMS)))
+    sdt = _KAPPA_FORMS[form]
     nm, nq = p + 1, p + 2
     if u.numel() % nm ** 3:
         raise ValueError("u must hold whole elements of %d values" % nm ** 3)
     ne = u.numel() // nm ** 3
     _check("basis", basis, nm * nq)
     _check("dbasis", dbasis, nq * nq)
-    _check("G", G, ne * 6 * nq ** 3)
-    _check("u", u, ne * nm ** 3)
-    return ne, nm, nq
+    _check("G", G, ne * 6 * nq ** 3, sdt)
+    _check("u", u, ne * nm ** 3, sdt)
+    return ne, nm, nq, sdt
 
 
-def bk3_kappa(p, basis, dbasis, G, kappa, u, out=None):
+def bk3_kappa(p, basis, dbasis, G, kappa, u, out=None, form="f64"):
     """v = A(kappa) u: BK3 with each Gauss point's six factors scaled by
-    kappa(e, p, q, r) (FP64)."""
-    ne, nm, nq = _bk3_common(p, basis, dbasis, G, u)
-    _check("kappa", kappa, ne * nq ** 3)
+    kappa(e, p, q, r).  form "f64": every array FP64; "f32w": u, G and kappa
+    stored in FP32, FP64 arithmetic; v FP64 in both."""
+    ne, nm, nq, sdt = _bk3_common(p, basis, dbasis, G, u, form)
+    _check("kappa", kappa, ne * nq ** 3, sdt)
     if out is None:
-        out = torch.empty_like(u)
+        out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nm ** 3)
     with torch.cuda.device(u.device):
-        _lib.call("bkx_bk3_apply_kappa_f64", p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(),
+        _lib.call("bkx_bk3_apply_kappa_" + form, p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(),
                   kappa.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
     return out
 
 
-def bk3_adjoint(p, basis, dbasis, G, kappa, u, b):
+def bk3_adjoint(p, basis, dbasis, G, kappa, u, b, form="f64"):
     """For L = 1/2 |A(kappa) u - b|^2: returns (L, dL/du, dL/dkappa, loss per
-    element), L as a 0-d device tensor summed in a fixed order."""
-    ne, nm, nq = _bk3_common(p, basis, dbasis, G, u)
-    _check("kappa", kappa, ne * nq ** 3)
-    _check("b", b, ne * nm ** 3)
-    dldu = torch.empty_like(u)
-    dldk = torch.empty_like(kappa)
+    element), L as a 0-d device tensor summed in a fixed order.  form as for
+    bk3_kappa (b in the storage type of u); the results are FP64."""
+    ne, nm, nq, sdt = _bk3_common(p, basis, dbasis, G, u, form)
+    _check("kappa", kappa, ne * nq ** 3, sdt)
+    _check("b", b, ne * nm ** 3, sdt)
+    dldu = torch.empty(u.shape, dtype=torch.float64, device=u.device)
+    dldk = torch.empty(kappa.shape, dtype=torch.float64, device=u.device)
     loss_e = torch.empty(ne, dtype=torch.float64, device=u.device)
     L = torch.empty((), dtype=torch.float64, device=u.device)
     with torch.cuda.device(u.device):
-        _lib.call("bkx_bk3_adjoint_f64", p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), kappa.data_ptr(),
-                  u.data_ptr(), b.data_ptr(), dldu.data_ptr(), dldk.data_ptr(), loss_e.data_ptr(), _stream(u))
+        _lib.call("bkx_bk3_adjoint_" + form, p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(),
+                  kappa.data_ptr(), u.data_ptr(), b.data_ptr(), dldu.data_ptr(), dldk.data_ptr(), loss_e.data_ptr(),
+                  _stream(u))
         _lib.call("bkx_sum_f64", loss_e.data_ptr(), ne, L.data_ptr(), _stream(u))
     return L, dldu, dldk, loss_e
 
 
-def bk3_vjp(p, basis, dbasis, G, kappa, u, w):
+def bk3_vjp(p, basis, dbasis, G, kappa, u, w, form="f64"):
     """Vector-Jacobian product of v = A(kappa) u with cotangent w:
-    (A(kappa) w, (grad w) . G (grad u))."""
-    ne, nm, nq = _bk3_common(p, basis, dbasis, G, u)
-    _check("kappa", kappa, ne * nq ** 3)
-    _check("w", w, ne * nm ** 3)
-    gu = torch.empty_like(u)
-    gk = torch.empty_like(kappa)
+    (A(kappa) w, (grad w) . G (grad u)).  form as for bk3_kappa (w in the
+    storage type of u); the results are FP64."""
+    ne, nm, nq, sdt = _bk3_common(p, basis, dbasis, G, u, form)
+    _check("kappa", kappa, ne * nq ** 3, sdt)
+    _check("w", w, ne * nm ** 3, sdt)
+    gu = torch.empty(u.shape, dtype=torch.float64, device=u.device)
+    gk = torch.empty(kappa.shape, dtype=torch.float64, device=u.device)
     with torch.cuda.device(u.device):
-        _lib.call("bkx_bk3_vjp_f64", p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), kappa.data_ptr(),
+        _lib.call("bkx_bk3_vjp_" + form, p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), kappa.data_ptr(),
                   u.data_ptr(), w.data_ptr(), gu.data_ptr(), gk.data_ptr(), _stream(u))
     return gu, gk

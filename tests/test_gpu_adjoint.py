@@ -129,3 +129,44 @@ def test_autograd_loss_matches_adjoint():
     assert abs(L.item() - La.item()) <= 1e-13 * La.item()
     assert ((uu.grad - dldu).abs().max() / dldu.abs().max()).item() <= 1e-13
     assert ((kk.grad - dldk).abs().max() / dldk.abs().max()).item() <= 1e-13
+
+
+@pytest.mark.parametrize("p", range(1, 9))
+def test_f32w_against_oracle(p):
+    """FP32-storage forms: u, b (w), kappa and G rounded to FP32 are exact
+    FP64 inputs of the oracle, so the FP64 arithmetic of the _f32w entry
+    points must match it to the FP64 bar."""
+    E = 7
+    B, D, G, kappa, u, b = _host_case(p, E, seed=70 + p)
+    G, kappa, u, b = (x.astype(np.float32) for x in (G, kappa, u, b))
+    d = lambda x: x.astype(np.float64)
+    f = lambda x: _t(x).float()
+    nq = p + 2
+    Lr, r, dldu_r, dldk_r = O.bk3_adjoint(nq, B, D, d(G), d(kappa), d(u), d(b))
+    v = ops.bk3_kappa(p, _t(B), _t(D), f(G), f(kappa), f(u), form="f32w").cpu().numpy()
+    assert O.rel_maxdiff(v, O.bk3_kappa(nq, B, D, d(G), d(kappa), d(u))) <= 1e-13
+    L, dldu, dldk, _ = ops.bk3_adjoint(p, _t(B), _t(D), f(G), f(kappa), f(u), f(b), form="f32w")
+    assert dldu.dtype == torch.float64 and dldk.dtype == torch.float64
+    assert abs(L.item() - Lr) <= 1e-13 * Lr
+    assert O.rel_maxdiff(dldu.cpu().numpy(), dldu_r) <= 1e-13
+    assert O.rel_maxdiff(dldk.cpu().numpy(), dldk_r) <= 1e-13
+    w = np.random.default_rng(p).uniform(-1, 1, u.size).astype(np.float32)
+    _, _, gu_r, gk_r = O.bk3_adjoint(nq, B, D, d(G), d(kappa), d(u), O.bk3_kappa(nq, B, D, d(G), d(kappa), d(u)) - d(w))
+    gu, gk = ops.bk3_vjp(p, _t(B), _t(D), f(G), f(kappa), f(u), f(w), form="f32w")
+    assert O.rel_maxdiff(gu.cpu().numpy(), gu_r) <= 1e-13
+    assert O.rel_maxdiff(gk.cpu().numpy(), gk_r) <= 1e-12
+
+
+def test_f32w_full_size_matches_f64_on_same_inputs():
+    """The adjoint configuration (p = 6, E = 16384) in the FP32-storage form
+    against the FP64 entry point on the same FP32-rounded inputs."""
+    p, E = 6, 16384
+    B, D, G, kappa, u, b, _ = _device_case(p, E, seed=7)
+    G, kappa, u, b = (x.float() for x in (G, kappa, u, b))
+    L64, dldu64, dldk64, _ = ops.bk3_adjoint(p, B, D, G.double(), kappa.double(), u.double(), b.double())
+    L32, dldu32, dldk32, _ = ops.bk3_adjoint(p, B, D, G, kappa, u, b, form="f32w")
+    torch.cuda.synchronize()
+    rel = lambda x, r: float(((x - r).abs().max() / r.abs().max()).item())
+    assert abs(L32.item() - L64.item()) <= 1e-13 * L64.item()
+    assert rel(dldu32, dldu64) <= 1e-13
+    assert rel(dldk32, dldk64) <= 1e-13
