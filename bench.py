@@ -109,6 +109,44 @@ def bk1_workload(p=3, E=4096):
     return dict(fn=fn, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + E * q ** 3), p=p, E=E)
 
 
+def geometry_lines(steps, warmup, p5=7, p3=7, E=65536):
+    """Geometric factors from nodal coordinates on the 64 x 32 x 32 perturbed
+    mesh: BK5's G at the GLL nodes (p = 7) and BK3's G with BK1's JxW at the
+    Gauss points (p = 7), by the HIP kernels; bytes are the coordinates read
+    (24 per node) and the factors written.  Beside them, the same mesh's
+    factors by torch's closed-form Jacobian of the perturbed map."""
+    import torch
+    from omp_bk_amd import geometry as geo
+    from omp_bk_amd import ops
+    dev = "cuda:0"
+    out = {}
+    n = p5 + 1
+    c5 = geo.nodal_coords(p5, E, device=dev)
+    D5 = torch.as_tensor(geo.bk5_basis(p5), device=dev)
+    w5 = torch.as_tensor(geo.gll(n)[1], device=dev)
+    G5 = torch.empty((E, n, n, 6, n), dtype=torch.float64, device=dev)
+    ms, _ = time_steps(lambda: ops.geo_bk5(p5, D5, w5, c5, out=G5), steps, warmup)
+    nb = E * n ** 3 * (24 + 48)
+    out["geo_bk5_p7"] = {"value": round(E * n ** 3 / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
+                         "bytes_per_dof": 72, "achieved_GBs": round(nb / (ms * 1e-3) * 1e-9, 1)}
+    ms_t, _ = time_steps(lambda: geo.geometric_factors(5, p5, E, device=dev), 3, 1)
+    out["geo_bk5_p7"]["torch_closed_form_ms"] = round(ms_t, 3)
+    del c5, G5
+    nm, nq = p3 + 1, p3 + 2
+    c3 = geo.nodal_coords(p3, E, device=dev)
+    B3, D3 = (torch.as_tensor(a, device=dev) for a in geo.bk13_bases(p3))
+    w3 = torch.as_tensor(geo.gauss(nq)[1], device=dev)
+    G3 = torch.empty((E, 6, nq, nq, nq), dtype=torch.float64, device=dev)
+    J3 = torch.empty((E, nq, nq, nq), dtype=torch.float64, device=dev)
+    ms, _ = time_steps(lambda: ops.geo_bk3(p3, B3, D3, w3, c3, G=G3, JxW=J3), steps, warmup)
+    nb = E * (24 * nm ** 3 + 56 * nq ** 3)
+    out["geo_bk3_p7"] = {"value": round(E * nm ** 3 / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
+                         "bytes_per_dof": round(nb / (E * nm ** 3), 2), "achieved_GBs": round(nb / (ms * 1e-3) * 1e-9, 1)}
+    del c3, G3, J3
+    torch.cuda.empty_cache()
+    return out
+
+
 def rate(w, ms):
     """GDoF/s, algorithmic GB/s and fraction of the 8 TB/s peak."""
     gbs = w["bytes"] / (ms * 1e-3) * 1e-9
@@ -350,6 +388,7 @@ def main():
         lines[name] = rate(w, ms)
         del w
     lines.update(adjoint_config(a.steps, a.warmup))
+    lines.update(geometry_lines(a.steps, a.warmup))
     torch.cuda.empty_cache()
 
     mix = mixed_batch(a.steps, a.warmup)

@@ -197,6 +197,31 @@ int bkx_bk3_vjp_f32w(int p, size_t nelmt, const double* basis, const double* dba
 int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
+/* Geometric factors from element nodal coordinates (the step before the    */
+/* operators; no reference counterpart: the reference's operators take     */
+/* synthetic factors, include/bk_common.h:92-107).                          */
+/*                                                                          */
+/* coords(e, c, i, j, k): x, y, z (c = 0, 1, 2) at the element's n^3 GLL    */
+/* nodes, n = p + 1.  J = d x / d xi of their degree-p interpolant, xi_1,   */
+/* xi_2, xi_3 the directions of the first, second, third tensor index;      */
+/* G = w detJ J^-1 J^-T (rr rs rt ss st tt) and JxW = w detJ, w the        */
+/* product of the three 1D weights.  FP64.                                  */
+/* ---------------------------------------------------------------------- */
+
+/* At the q = p + 2 Gauss points: G factor-major, as bkx_bk3_apply_f64 reads
+ * it, and JxW(e, p, q, r), as bkx_bk1_apply_f64 reads it; either may be NULL
+ * (not both).  basis and dbasis as for bkx_bk3_apply_f64, qweights the q
+ * Gauss weights.  p = 1..8. */
+int bkx_geo_bk3_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* qweights,
+                    const double* coords, double* G, double* JxW, bkx_stream_t stream);
+
+/* At the n GLL nodes (collocated): G interleaved, G(e, i, j, f, k), as
+ * bkx_bk5_apply_f64 reads it.  dbasis as for bkx_bk5_apply_f64, qweights the
+ * n GLL weights.  p = 1..7. */
+int bkx_geo_bk5_f64(int p, size_t nelmt, const double* dbasis, const double* qweights, const double* coords,
+                    double* G, bkx_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
 /* Mixed-order BK3 batches (no reference counterpart: the reference runs    */
 /* one order per binary, include/bk_common.h:143-178).                     */
 /*                                                                          */

@@ -34,6 +34,8 @@ _SIGNATURES = {
     "bkx_bk3_adjoint_f32w": [_c_int, _c_size] + [_vp] * 10,
     "bkx_bk3_vjp_f32w": [_c_int, _c_size] + [_vp] * 9,
     "bkx_sum_f64": [_vp, _c_size, _vp, _vp],
+    "bkx_geo_bk3_f64": [_c_int, _c_size] + [_vp] * 7,
+    "bkx_geo_bk5_f64": [_c_int, _c_size] + [_vp] * 5,
     "bkx_probe_copy": [_vp, _vp, _c_size, _vp],
     "bkx_probe_read_mix": [_c_int, _vp, _vp, _c_size, _vp],
     "bkx_tune_count": [_c_int, _c_int],

@@ -29,6 +29,10 @@ int bk3_adjoint_f32w(int p, size_t nelmt, const double* B, const double* D, cons
 int bk3_vjp_f32w(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
                  const float* u, const float* w, double* gu, double* gk, hipStream_t s);
 int sum_f64(const double* x, size_t n, double* out, hipStream_t s);
+int geo_bk3_f64(int p, size_t nelmt, const double* B, const double* D, const double* wq, const double* coords,
+                double* G, double* JxW, hipStream_t s);
+int geo_bk5_f64(int p, size_t nelmt, const double* D, const double* wq, const double* coords, double* G,
+                hipStream_t s);
 int bk5_apply_bf16s(int p, size_t nelmt, const double* D, const float* G, const float* u, float* v, hipStream_t s);
 int bk5_variants();
 int bk5_apply_f64(int variant, int p, size_t nelmt, const double* D, const double* G, const double* u, double* v,
@@ -445,6 +449,36 @@ int bkx_bk3_vjp_f32w(int p, size_t nelmt, const double* basis, const double* dba
         return BKX_ERROR_NULL;
     }
     return bkx::bk3_vjp_f32w(p, nelmt, basis, dbasis, G, kappa, u, w, grad_u, grad_kappa, as_stream(stream));
+}
+
+int bkx_geo_bk3_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* qweights,
+                    const double* coords, double* G, double* JxW, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !qweights || !coords || (!G && !JxW)) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::geo_bk3_f64(p, nelmt, basis, dbasis, qweights, coords, G, JxW, as_stream(stream));
+}
+
+int bkx_geo_bk5_f64(int p, size_t nelmt, const double* dbasis, const double* qweights, const double* coords,
+                    double* G, bkx_stream_t stream)
+{
+    if (!order_ok(5, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!dbasis || !qweights || !coords || !G) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::geo_bk5_f64(p, nelmt, dbasis, qweights, coords, G, as_stream(stream));
 }
 
 int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream)

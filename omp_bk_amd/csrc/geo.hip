@@ -1,0 +1,223 @@
+// geo.hip -- geometric factors from element nodal coordinates, in FP64 on
+// gfx950 (the step before the operators: the reference feeds its operators
+// synthetic factors, include/bk_common.h:92-107, so there is no reference
+// kernel; the factors are those its harness describes, w detJ J^-1 J^-T and
+// w detJ).
+//
+// coords(e, c, i, j, k): the element's coordinates x, y, z (c = 0, 1, 2) at
+// its n^3 GLL nodes, n = p + 1.  J(c, d) = d x_c / d xi_d of the degree-p
+// interpolant of the coordinates, with xi_1, xi_2, xi_3 the directions of
+// the first, second and third tensor index.  At each point, with weight
+// w = w_a w_b w_c:
+//   JxW = w detJ,   G = w detJ J^-1 J^-T = w adj(J) adj(J)^T / detJ
+// (adj the adjugate; factors rr rs rt ss st tt).
+//
+// BK3 / BK1 (Gauss points, q = p + 2): the reference gradient of each
+// coordinate component, computed by the phases of the transposing BK3
+// mapping (bk3_phases.hpp: interpolate with B, differentiate with the Gauss
+// collocation derivative D), one element per workgroup, J kept in the LDS;
+// G is written factor-major, G(e, f, p, q, r), JxW as JxW(e, p, q, r).
+// BK5 (GLL nodes, collocated): J by the collocated derivative D(i, n) along
+// each direction from the coordinates staged in the LDS; G is written
+// interleaved, G(e, i, j, f, k).
+//
+// The coordinates are differentiated relative to the element's first node,
+// x - x(0, 0, 0): the derivative matrices annihilate constants, and this
+// removes the cancellation of differentiating absolute coordinates (up to
+// |x| / h times larger than the derivatives), which cost two to three digits
+// of J at p >= 7.  The subtraction is exact wherever x(0, 0, 0) is more
+// than the element size from the origin (Sterbenz), and its rounding is
+// relative to the element size elsewhere.
+#include "bk3_phases.hpp"
+#include "bkx_launch.hpp"
+
+namespace bkx {
+namespace {
+
+// G = w adj(J) adj(J)^T / detJ and w detJ from J (row c = d x_c / d xi)
+__device__ __forceinline__ void factors_of(const double (&J)[3][3], double w, double (&g)[6], double& wdet)
+{
+    // adjugate A = cofactor^T: A(d, c) = cofactor(c, d), so that J A = det I
+    double A[3][3];
+    A[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    A[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    A[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    A[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    A[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    A[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    A[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    A[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    A[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    const double det = J[0][0] * A[0][0] + J[0][1] * A[1][0] + J[0][2] * A[2][0];
+    wdet = w * det;
+    // J^-1 = A / det: G(a, b) = w det sum_c J^-1(a, c) J^-1(b, c) = (w / det) sum_c A(a, c) A(b, c)
+    const double s = w / det;
+    constexpr int fa[6] = {0, 0, 0, 1, 1, 2};
+    constexpr int fb[6] = {0, 1, 2, 1, 2, 2};
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        const int a = fa[f], b = fb[f];
+        g[f] = s * (A[a][0] * A[b][0] + A[a][1] * A[b][1] + A[a][2] * A[b][2]);
+    }
+}
+
+template <int NM, int NQ>
+struct GeoShape {
+    static constexpr int nq2 = NQ * NQ;
+    static constexpr int nq3 = nq2 * NQ;
+    static constexpr int threads = ((nq2 + kWave - 1) / kWave) * kWave;
+};
+
+// BK3 / BK1 points.  One element per workgroup; lane l < q^2 is the (q, r)
+// lane of the Gauss-point plane, holding column p.
+template <int NM, int NQ>
+__global__ __launch_bounds__((GeoShape<NM, NQ>::threads)) void geo_gauss_kernel(
+    size_t nelmt, const double* __restrict__ Bm, const double* __restrict__ Dm, const double* __restrict__ wq,
+    const double* __restrict__ coords, double* __restrict__ G, double* __restrict__ JxW)
+{
+    using E = bk3p::Elem<NM, NQ>;
+    using S = GeoShape<NM, NQ>;
+    __shared__ double s_box[3 * E::box];
+    __shared__ double s_J[9][S::nq3];   // J(c, d) at point (p, lane)
+
+    const size_t e = blockIdx.x;
+    const int l = threadIdx.x;
+    const bool in_elem = l < S::nq2;
+    E el;
+    el.init(s_box, in_elem ? l : 0, in_elem);
+    const bool modal = in_elem && l < E::nm2;
+#pragma unroll 1
+    for (int c = 0; c < 3; ++c) {
+        double x[NM];
+        const double* xc = coords + (e * 3 + c) * E::nm3;
+        const double x0 = xc[0];
+        const double* xe = xc + (modal ? l : 0);
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            x[i] = modal ? xe[i * E::nm2] - x0 : 0.0;
+        }
+        double qr[NQ];
+        el.template grad<0>(Bm, Dm, x, qr);   // ends with the workgroup synchronised
+        if (in_elem) {
+#pragma unroll
+            for (int p = 0; p < NQ; ++p) {
+                const int xx = E::X(p, el.lq, el.lq2);
+                s_J[c * 3 + 0][p * S::nq2 + l] = qr[p];
+                s_J[c * 3 + 1][p * S::nq2 + l] = el.B[xx];
+                s_J[c * 3 + 2][p * S::nq2 + l] = el.C[xx];
+            }
+        }
+        // the next gradient writes box A, then box B only after a barrier,
+        // so this lane's reads of B and C above come first
+    }
+    if (!in_elem) {
+        return;
+    }
+    const double wqr = wq[el.lq] * wq[el.lq2];
+#pragma unroll 1
+    for (int p = 0; p < NQ; ++p) {
+        double J[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                J[c][d] = s_J[c * 3 + d][p * S::nq2 + l];
+            }
+        }
+        double g[6], wdet;
+        factors_of(J, wq[p] * wqr, g, wdet);
+        if (G) {
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                G[(e * 6 + f) * S::nq3 + p * S::nq2 + l] = g[f];
+            }
+        }
+        if (JxW) {
+            JxW[e * S::nq3 + p * S::nq2 + l] = wdet;
+        }
+    }
+}
+
+// BK5 points (GLL nodes, collocated).  One element per workgroup; lane
+// l = (j, k) < n^2 loops over i.
+template <int N>
+__global__ __launch_bounds__((GeoShape<N, N>::threads)) void geo_gll_kernel(
+    size_t nelmt, const double* __restrict__ D, const double* __restrict__ wq, const double* __restrict__ coords,
+    double* __restrict__ G)
+{
+    constexpr int n2 = N * N;
+    constexpr int n3 = n2 * N;
+    __shared__ double sx[3 * n3];
+    const size_t e = blockIdx.x;
+    const int l = threadIdx.x;
+    for (int t = l; t < 3 * n3; t += blockDim.x) {
+        const double* xc = coords + e * 3 * n3 + (t / n3) * n3;
+        sx[t] = xc[t % n3] - xc[0];
+    }
+    __syncthreads();
+    if (l >= n2) {
+        return;
+    }
+    const int j = l / N;
+    const int k = l - j * N;
+    const auto Dr = launder<1>(D);
+#pragma unroll 1
+    for (int i = 0; i < N; ++i) {
+        double J[3][3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const double* xc = sx + c * n3;
+            double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+#pragma unroll
+            for (int m = 0; m < N; ++m) {
+                d0 += Dr[i * N + m] * xc[(m * N + j) * N + k];
+                d1 += Dr[j * N + m] * xc[(i * N + m) * N + k];
+                d2 += Dr[k * N + m] * xc[(i * N + j) * N + m];
+            }
+            J[c][0] = d0;
+            J[c][1] = d1;
+            J[c][2] = d2;
+        }
+        double g[6], wdet;
+        factors_of(J, wq[i] * wq[j] * wq[k], g, wdet);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            G[e * 6 * n3 + ((i * N + j) * 6 + f) * N + k] = g[f];
+        }
+    }
+}
+
+}  // namespace
+
+int geo_bk3_f64(int p, size_t nelmt, const double* B, const double* D, const double* wq, const double* coords,
+                double* G, double* JxW, hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto P) {
+        constexpr int NM = decltype(P)::value + 1;
+        constexpr int NQ = NM + 1;
+        const unsigned grid = grid_for(nelmt, 1);
+        if (grid == 0) {
+            return int(BKX_ERROR_SIZE);
+        }
+        hipLaunchKernelGGL((geo_gauss_kernel<NM, NQ>), dim3(grid), dim3(GeoShape<NM, NQ>::threads), 0, s, nelmt, B, D,
+                           wq, coords, G, JxW);
+        return launch_status();
+    });
+}
+
+int geo_bk5_f64(int p, size_t nelmt, const double* D, const double* wq, const double* coords, double* G,
+                hipStream_t s)
+{
+    return with_order<1, 7>(p, [&](auto P) {
+        constexpr int N = decltype(P)::value + 1;
+        const unsigned grid = grid_for(nelmt, 1);
+        if (grid == 0) {
+            return int(BKX_ERROR_SIZE);
+        }
+        hipLaunchKernelGGL((geo_gll_kernel<N>), dim3(grid), dim3(GeoShape<N, N>::threads), 0, s, nelmt, D, wq, coords,
+                           G);
+        return launch_status();
+    });
+}
+
+}  // namespace bkx

@@ -267,6 +267,28 @@ def geometric_factors(kind, p, nelmt, amp=0.1, shape=None, chunk=8192, coords=Fa
     return (out, xyz) if coords else out
 
 
+def nodal_coords(p, nelmt, amp=0.1, shape=None, device=None):
+    """Coordinates (E, 3, n, n, n) of the perturbed box mesh at each element's
+    n = p + 1 GLL nodes (the input of the geometric-factor kernels): the
+    affine box map followed by x -> x + delta(x) (1, 1, 1).  numpy, or a
+    float64 torch tensor on ``device``."""
+    shape = shape or box_shape(nelmt)
+    if shape[0] * shape[1] * shape[2] != nelmt:
+        raise ValueError("mesh shape does not hold nelmt elements")
+    pts, _ = gll(p + 1)
+    if device is None:
+        xp, dev = np, None
+    else:
+        import torch as xp
+        dev = device
+    X, Y, Z = _axes(shape, pts, 0, nelmt, xp, dev)
+    d, _ = displacement(X, Y, Z, amp, 1.0 / max(shape), xp)
+    comps = [X + d, Y + d, Z + d]
+    if xp is np:
+        return np.stack([np.broadcast_to(c, d.shape) for c in comps], axis=1).copy()
+    return xp.stack([c.expand(d.shape) for c in comps], dim=1).contiguous()
+
+
 def random_u(nvalues, seed=0):
     """i.i.d. uniform[-1, 1] FP64 values from numpy's PCG64 with ``seed``."""
     return np.random.default_rng(seed).uniform(-1.0, 1.0, nvalues)

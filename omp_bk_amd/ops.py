@@ -214,3 +214,49 @@ def bk3_vjp(p, basis, dbasis, G, kappa, u, w, form="f64"):
         _lib.call("bkx_bk3_vjp_" + form, p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), kappa.data_ptr(),
                   u.data_ptr(), w.data_ptr(), gu.data_ptr(), gk.data_ptr(), _stream(u))
     return gu, gk
+
+
+def geo_bk3(p, basis, dbasis, qweights, coords, G=None, JxW=None):
+    """Geometric factors at the q = p + 2 Gauss points from element nodal
+    coordinates coords(e, 3, n, n, n) at the GLL nodes: G (E, 6, q, q, q)
+    factor-major (BK3) and JxW (E, q, q, q) (BK1), both FP64; pass the output
+    tensors, or None for a new one.  Returns (G, JxW)."""
+    _order_ok(3, p)
+    nm, nq = p + 1, p + 2
+    if coords.numel() % (3 * nm ** 3):
+        raise ValueError("coords must hold whole elements of 3 x %d values" % nm ** 3)
+    ne = coords.numel() // (3 * nm ** 3)
+    _check("basis", basis, nm * nq)
+    _check("dbasis", dbasis, nq * nq)
+    _check("qweights", qweights, nq)
+    _check("coords", coords, ne * 3 * nm ** 3)
+    if G is None:
+        G = torch.empty((ne, 6, nq, nq, nq), dtype=torch.float64, device=coords.device)
+    if JxW is None:
+        JxW = torch.empty((ne, nq, nq, nq), dtype=torch.float64, device=coords.device)
+    _check("G", G, ne * 6 * nq ** 3)
+    _check("JxW", JxW, ne * nq ** 3)
+    with torch.cuda.device(coords.device):
+        _lib.call("bkx_geo_bk3_f64", p, ne, basis.data_ptr(), dbasis.data_ptr(), qweights.data_ptr(),
+                  coords.data_ptr(), G.data_ptr(), JxW.data_ptr(), _stream(coords))
+    return G, JxW
+
+
+def geo_bk5(p, dbasis, qweights, coords, out=None):
+    """BK5 geometric factors G (E, n, n, 6, n) interleaved at the n = p + 1
+    GLL nodes from element nodal coordinates coords(e, 3, n, n, n), FP64."""
+    _order_ok(5, p)
+    n = p + 1
+    if coords.numel() % (3 * n ** 3):
+        raise ValueError("coords must hold whole elements of 3 x %d values" % n ** 3)
+    ne = coords.numel() // (3 * n ** 3)
+    _check("dbasis", dbasis, n * n)
+    _check("qweights", qweights, n)
+    _check("coords", coords, ne * 3 * n ** 3)
+    if out is None:
+        out = torch.empty((ne, n, n, 6, n), dtype=torch.float64, device=coords.device)
+    _check("out", out, ne * 6 * n ** 3)
+    with torch.cuda.device(coords.device):
+        _lib.call("bkx_geo_bk5_f64", p, ne, dbasis.data_ptr(), qweights.data_ptr(), coords.data_ptr(),
+                  out.data_ptr(), _stream(coords))
+    return out
