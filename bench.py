@@ -147,6 +147,40 @@ def geometry_lines(steps, warmup, p5=7, p3=7, E=65536):
     return out
 
 
+def assembled_bk5(steps, warmup, p=7, E=65536):
+    """The assembled BK5 operator v_L = R^T A R u_L (continuous GLL space) on
+    the 64 x 32 x 32 perturbed mesh, p = 7: gather, the element operator and
+    the deterministic scatter, timed together and apart.  Rates per global
+    (L-vector) DoF and per element (E-vector) DoF."""
+    import torch
+    from omp_bk_amd import geometry as geo
+    from omp_bk_amd import ops
+    from omp_bk_amd.restriction import Restriction
+    dev = "cuda:0"
+    n = p + 1
+    off, lsize = geo.box_offsets(p, geo.box_shape(E))
+    R = Restriction(off, lsize)
+    D = torch.as_tensor(geo.bk5_basis(p), device=dev)
+    G = geo.geometric_factors(5, p, E, device=dev).reshape(-1)
+    uL = torch.rand(lsize, dtype=torch.float64, device=dev) * 2 - 1
+    uE = torch.empty(E * n ** 3, dtype=torch.float64, device=dev)
+    vE = torch.empty_like(uE)
+    vL = torch.empty_like(uL)
+
+    def apply():
+        R.gather(uL, out=uE)
+        ops.bk5(p, D, G, uE, out=vE)
+        R.scatter(vE, out=vL)
+    ms, _ = time_steps(apply, steps, warmup)
+    ms_g, _ = time_steps(lambda: R.gather(uL, out=uE), steps, warmup)
+    ms_s, _ = time_steps(lambda: R.scatter(vE, out=vL), steps, warmup)
+    R.close()
+    return {"bk5_p7_assembled": {"value": round(lsize / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s (global)",
+                                 "edofs_per_s": round(E * n ** 3 / (ms * 1e-3) * 1e-9, 3), "ms_per_step": round(ms, 5),
+                                 "global_dofs": lsize, "element_dofs": E * n ** 3, "gather_ms": round(ms_g, 5),
+                                 "scatter_ms": round(ms_s, 5)}}
+
+
 def rate(w, ms):
     """GDoF/s, algorithmic GB/s and fraction of the 8 TB/s peak."""
     gbs = w["bytes"] / (ms * 1e-3) * 1e-9
@@ -389,6 +423,7 @@ def main():
         del w
     lines.update(adjoint_config(a.steps, a.warmup))
     lines.update(geometry_lines(a.steps, a.warmup))
+    lines.update(assembled_bk5(a.steps, a.warmup))
     torch.cuda.empty_cache()
 
     mix = mixed_batch(a.steps, a.warmup)

@@ -222,6 +222,28 @@ int bkx_geo_bk5_f64(int p, size_t nelmt, const double* dbasis, const double* qwe
                     double* G, bkx_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
+/* Element restriction (no reference counterpart: the reference applies     */
+/* its operators to E-vectors only).  offsets[e * elemsize + t] is the     */
+/* global node (0 .. lsize - 1) of element e's local node t, as libCEED's   */
+/* element-restriction offsets; gather: uE = R uL, scatter: vL = R^T vE.    */
+/* R^T A R is then the assembled (continuous Galerkin) operator.           */
+/* ---------------------------------------------------------------------- */
+
+typedef struct bkx_restrict bkx_restrict;
+
+/* Copies the host array offsets (nelmt * elemsize) to the device with its
+ * transpose (each node's E-vector positions in ascending order).  Checks
+ * every offset against lsize. */
+int bkx_restrict_create(size_t nelmt, int elemsize, const int32_t* offsets, size_t lsize, bkx_restrict** plan);
+int bkx_restrict_destroy(bkx_restrict* plan);
+int bkx_restrict_info(const bkx_restrict* plan, size_t* nelmt, int* elemsize, size_t* lsize);
+/* uE[x] = uL[offsets[x]] (uL: lsize, uE: nelmt * elemsize values). */
+int bkx_restrict_gather_f64(const bkx_restrict* plan, const double* uL, double* uE, bkx_stream_t stream);
+/* vL[g] = sum of vE[x] over offsets[x] = g, x ascending: deterministic, and
+ * equal bit for bit to the sequential accumulation (vL is overwritten). */
+int bkx_restrict_scatter_f64(const bkx_restrict* plan, const double* vE, double* vL, bkx_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
 /* Mixed-order BK3 batches (no reference counterpart: the reference runs    */
 /* one order per binary, include/bk_common.h:143-178).                     */
 /*                                                                          */

@@ -36,6 +36,11 @@ _SIGNATURES = {
     "bkx_sum_f64": [_vp, _c_size, _vp, _vp],
     "bkx_geo_bk3_f64": [_c_int, _c_size] + [_vp] * 7,
     "bkx_geo_bk5_f64": [_c_int, _c_size] + [_vp] * 5,
+    "bkx_restrict_create": [_c_size, _c_int, _vp, _c_size, _vp],
+    "bkx_restrict_destroy": [_vp],
+    "bkx_restrict_info": [_vp, _vp, _vp, _vp],
+    "bkx_restrict_gather_f64": [_vp, _vp, _vp, _vp],
+    "bkx_restrict_scatter_f64": [_vp, _vp, _vp, _vp],
     "bkx_probe_copy": [_vp, _vp, _c_size, _vp],
     "bkx_probe_read_mix": [_c_int, _vp, _vp, _c_size, _vp],
     "bkx_tune_count": [_c_int, _c_int],

@@ -1,0 +1,198 @@
+// restrict.hip -- element restriction around the element-local operators:
+// the L-vector (one value per global node, continuous across element
+// faces) to the E-vector the operators read (gather, u_E = R u_L) and back
+// (scatter-add, v_L = R^T v_E), so that R^T A_E R is the assembled operator
+// of a continuous Galerkin discretisation.  No reference counterpart: the
+// reference applies its operators to E-vectors only.
+//
+// offsets[x], x = e * elemsize + t, is the global node of element e's local
+// node t (t = (i n + j) n + k), as libCEED's element restriction offsets.
+// The plan holds them on the device together with their transpose: for each
+// global node g, the E-vector positions x with offsets[x] = g in ascending
+// order (a CSR structure).  The scatter sums each node's contributions in
+// that order, one thread per node: deterministic, with no atomics, and
+// bit-identical to the sequential loop v_L[offsets[x]] += v_E[x] over x
+// ascending.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <new>
+#include <vector>
+
+#include "bkx.h"
+#include "bkx_launch.hpp"
+
+struct bkx_restrict {
+    size_t nelmt = 0;
+    int elemsize = 0;
+    size_t lsize = 0;
+    int32_t* offsets = nullptr;   // nelmt * elemsize
+    int32_t* row = nullptr;       // lsize + 1
+    int32_t* col = nullptr;       // nelmt * elemsize, E-vector positions by node
+};
+
+namespace bkx {
+namespace {
+
+constexpr int kThreads = 256;
+
+__global__ __launch_bounds__(kThreads) void gather_kernel(size_t n, const int32_t* __restrict__ offsets,
+                                                          const double* __restrict__ uL, double* __restrict__ uE)
+{
+    const size_t x = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (x < n) {
+        uE[x] = uL[offsets[x]];
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void scatter_kernel(size_t lsize, const int32_t* __restrict__ row,
+                                                           const int32_t* __restrict__ col,
+                                                           const double* __restrict__ vE, double* __restrict__ vL)
+{
+    const size_t g = size_t(blockIdx.x) * kThreads + threadIdx.x;
+    if (g < lsize) {
+        double s = 0.0;
+        for (int32_t t = row[g]; t < row[g + 1]; ++t) {
+            s += vE[col[t]];
+        }
+        vL[g] = s;
+    }
+}
+
+template <typename T>
+int upload(T** dst, const std::vector<T>& src)
+{
+    if (hipMalloc(reinterpret_cast<void**>(dst), std::max<size_t>(1, src.size()) * sizeof(T)) != hipSuccess) {
+        return BKX_ERROR_DEVICE;
+    }
+    if (!src.empty() && hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+        return BKX_ERROR_DEVICE;
+    }
+    return BKX_SUCCESS;
+}
+
+}  // namespace
+}  // namespace bkx
+
+extern "C" {
+
+int bkx_restrict_create(size_t nelmt, int elemsize, const int32_t* offsets, size_t lsize, bkx_restrict** plan)
+{
+    if (!plan || (nelmt > 0 && !offsets)) {
+        return BKX_ERROR_NULL;
+    }
+    *plan = nullptr;
+    if (elemsize < 1) {
+        return BKX_ERROR_ARG;
+    }
+    const size_t n = nelmt * size_t(elemsize);
+    if (n > size_t(INT32_MAX) || lsize > size_t(INT32_MAX) - 1) {
+        return BKX_ERROR_SIZE;
+    }
+    // counting sort of the E-vector positions by node (ascending within a node)
+    std::vector<int32_t> row(lsize + 1, 0);
+    for (size_t x = 0; x < n; ++x) {
+        const int32_t g = offsets[x];
+        if (g < 0 || size_t(g) >= lsize) {
+            return BKX_ERROR_ARG;
+        }
+        ++row[size_t(g) + 1];
+    }
+    for (size_t g = 0; g < lsize; ++g) {
+        row[g + 1] += row[g];
+    }
+    std::vector<int32_t> col(n);
+    std::vector<int32_t> next(row.begin(), row.end() - 1);
+    for (size_t x = 0; x < n; ++x) {
+        col[size_t(next[size_t(offsets[x])]++)] = int32_t(x);
+    }
+    auto* pl = new (std::nothrow) bkx_restrict;
+    if (!pl) {
+        return BKX_ERROR_DEVICE;
+    }
+    pl->nelmt = nelmt;
+    pl->elemsize = elemsize;
+    pl->lsize = lsize;
+    const std::vector<int32_t> off(offsets, offsets + n);
+    int st = bkx::upload(&pl->offsets, off);
+    if (st == BKX_SUCCESS) {
+        st = bkx::upload(&pl->row, row);
+    }
+    if (st == BKX_SUCCESS) {
+        st = bkx::upload(&pl->col, col);
+    }
+    if (st != BKX_SUCCESS) {
+        bkx_restrict_destroy(pl);
+        return st;
+    }
+    *plan = pl;
+    return BKX_SUCCESS;
+}
+
+int bkx_restrict_destroy(bkx_restrict* plan)
+{
+    if (!plan) {
+        return BKX_SUCCESS;
+    }
+    for (void* ptr : {static_cast<void*>(plan->offsets), static_cast<void*>(plan->row), static_cast<void*>(plan->col)}) {
+        if (ptr) {
+            (void)hipFree(ptr);
+        }
+    }
+    delete plan;
+    return BKX_SUCCESS;
+}
+
+int bkx_restrict_info(const bkx_restrict* plan, size_t* nelmt, int* elemsize, size_t* lsize)
+{
+    if (!plan) {
+        return BKX_ERROR_NULL;
+    }
+    if (nelmt) {
+        *nelmt = plan->nelmt;
+    }
+    if (elemsize) {
+        *elemsize = plan->elemsize;
+    }
+    if (lsize) {
+        *lsize = plan->lsize;
+    }
+    return BKX_SUCCESS;
+}
+
+int bkx_restrict_gather_f64(const bkx_restrict* plan, const double* uL, double* uE, bkx_stream_t stream)
+{
+    if (!plan || !uL || !uE) {
+        return BKX_ERROR_NULL;
+    }
+    const size_t n = plan->nelmt * size_t(plan->elemsize);
+    if (n == 0) {
+        return BKX_SUCCESS;
+    }
+    const unsigned grid = bkx::grid_for(n, bkx::kThreads);
+    if (grid == 0) {
+        return BKX_ERROR_SIZE;
+    }
+    hipLaunchKernelGGL(bkx::gather_kernel, dim3(grid), dim3(bkx::kThreads), 0, reinterpret_cast<hipStream_t>(stream),
+                       n, plan->offsets, uL, uE);
+    return bkx::launch_status();
+}
+
+int bkx_restrict_scatter_f64(const bkx_restrict* plan, const double* vE, double* vL, bkx_stream_t stream)
+{
+    if (!plan || !vE || !vL) {
+        return BKX_ERROR_NULL;
+    }
+    if (plan->lsize == 0) {
+        return BKX_SUCCESS;
+    }
+    const unsigned grid = bkx::grid_for(plan->lsize, bkx::kThreads);
+    if (grid == 0) {
+        return BKX_ERROR_SIZE;
+    }
+    hipLaunchKernelGGL(bkx::scatter_kernel, dim3(grid), dim3(bkx::kThreads), 0,
+                       reinterpret_cast<hipStream_t>(stream), plan->lsize, plan->row, plan->col, vE, vL);
+    return bkx::launch_status();
+}
+
+}  // extern "C"

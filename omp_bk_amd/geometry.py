@@ -289,6 +289,24 @@ def nodal_coords(p, nelmt, amp=0.1, shape=None, device=None):
     return xp.stack([c.expand(d.shape) for c in comps], dim=1).contiguous()
 
 
+def box_offsets(p, shape):
+    """Element-restriction offsets of the continuous degree-p space on an
+    nx x ny x nz box mesh: int32 (E, n^3), the global node of each element's
+    GLL node (i, j, k), numbered over the (nx p + 1) x (ny p + 1) x (nz p + 1)
+    grid with x fastest; and the number of global nodes."""
+    nx, ny, nz = shape
+    n = p + 1
+    NX, NY, NZ = nx * p + 1, ny * p + 1, nz * p + 1
+    e = np.arange(nx * ny * nz)
+    ex, ey, ez = e % nx, (e // nx) % ny, e // (nx * ny)
+    i = np.arange(n)
+    gx = (ex[:, None] * p + i[None, :])[:, :, None, None]
+    gy = (ey[:, None] * p + i[None, :])[:, None, :, None]
+    gz = (ez[:, None] * p + i[None, :])[:, None, None, :]
+    off = (gz * NY + gy) * NX + gx
+    return off.reshape(nx * ny * nz, n ** 3).astype(np.int32), NX * NY * NZ
+
+
 def random_u(nvalues, seed=0):
     """i.i.d. uniform[-1, 1] FP64 values from numpy's PCG64 with ``seed``."""
     return np.random.default_rng(seed).uniform(-1.0, 1.0, nvalues)

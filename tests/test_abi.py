@@ -1,9 +1,11 @@
 """The C ABI: libbkx.so loads, exports every function include/bkx.h declares,
 and rejects bad arguments before touching the device (no GPU needed)."""
+import ctypes
 import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from omp_bk_amd import _lib
@@ -48,6 +50,18 @@ def test_error_paths_without_device():
     assert [L.bkx_max_order(k) for k in (1, 3, 5)] == [8, 8, 7]
     assert L.bkx_max_order(2) == -1
     assert L.bkx_status_string(1) == b"unsupported polynomial order p"
+    # geometric factors and element restriction: checked before device work
+    assert L.bkx_geo_bk3_f64(9, 10, None, None, None, None, None, None, None) == 1
+    assert L.bkx_geo_bk5_f64(8, 10, None, None, None, None, None) == 1
+    assert L.bkx_geo_bk5_f64(7, 10, None, None, None, None, None) == 2
+    h = ctypes.c_void_p()
+    off = np.array([[0, 1, 2, 9]], dtype=np.int32)
+    ptr = off.ctypes.data_as(ctypes.c_void_p)
+    assert L.bkx_restrict_create(1, 4, None, 10, ctypes.byref(h)) == 2
+    assert L.bkx_restrict_create(1, 0, ptr, 10, ctypes.byref(h)) == 5
+    assert L.bkx_restrict_create(1, 4, ptr, 9, ctypes.byref(h)) == 5      # offset 9 >= lsize
+    assert L.bkx_restrict_gather_f64(None, None, None, None) == 2
+    assert L.bkx_restrict_scatter_f64(None, None, None, None) == 2
 
 
 def test_python_errors():
