@@ -293,7 +293,10 @@ def box_offsets(p, shape):
     """Element-restriction offsets of the continuous degree-p space on an
     nx x ny x nz box mesh: int32 (E, n^3), the global node of each element's
     GLL node (i, j, k), numbered over the (nx p + 1) x (ny p + 1) x (nz p + 1)
-    grid with x fastest; and the number of global nodes."""
+    grid with z fastest (the axis of the element's fastest index k, so that
+    consecutive global nodes are consecutive E-vector values within an
+    element and the gather and scatter read whole lines); and the number of
+    global nodes."""
     nx, ny, nz = shape
     n = p + 1
     NX, NY, NZ = nx * p + 1, ny * p + 1, nz * p + 1
@@ -303,7 +306,7 @@ def box_offsets(p, shape):
     gx = (ex[:, None] * p + i[None, :])[:, :, None, None]
     gy = (ey[:, None] * p + i[None, :])[:, None, :, None]
     gz = (ez[:, None] * p + i[None, :])[:, None, None, :]
-    off = (gz * NY + gy) * NX + gx
+    off = (gx * NY + gy) * NZ + gz
     return off.reshape(nx * ny * nz, n ** 3).astype(np.int32), NX * NY * NZ
 
 

@@ -17,7 +17,7 @@ def test_multiplicities(p, shape):
     m = R.scatter(off, np.ones(off.size), lsize)
     assert m.sum() == off.size and m.min() >= 1
     nx, ny, nz = shape
-    grid = m.reshape(nz * p + 1, ny * p + 1, nx * p + 1)
+    grid = m.reshape(nx * p + 1, ny * p + 1, nz * p + 1).transpose(2, 1, 0)   # indexed [gz, gy, gx]
 
     def share(n, g):   # elements along one axis sharing grid line g
         return 2 if (g % p == 0 and 0 < g < n * p) else 1
