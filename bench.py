@@ -106,7 +106,7 @@ def bk1_workload(p=3, E=4096):
     u = torch.rand(E * n ** 3, dtype=torch.float64, device=dev) * 2 - 1
     v = torch.empty_like(u)
     fn = lambda: ops.bk1(p, B, J, u, out=v)
-    return dict(fn=fn, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + E * q ** 3), p=p, E=E)
+    return dict(fn=fn, out=v, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + E * q ** 3), p=p, E=E)
 
 
 def geometry_lines(steps, warmup, p5=7, p3=7, E=65536):

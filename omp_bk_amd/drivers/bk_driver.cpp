@@ -30,6 +30,14 @@
 
 namespace {
 
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) {
+        std::cerr << what << ": " << hipGetErrorString(e) << "\n";
+        std::exit(1);
+    }
+}
+
 std::vector<double> test_basis(int rows, int cols)
 {
     std::vector<double> m(std::size_t(rows) * cols);
@@ -112,10 +120,10 @@ int main(int argc, char** argv)
     int status = apply();   // warm-up
     double best = std::numeric_limits<double>::max();
     for (int t = 0; t < ntests && status == BKX_SUCCESS; ++t) {
-        hipDeviceSynchronize();
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         const auto t0 = std::chrono::steady_clock::now();
         status = apply();
-        hipDeviceSynchronize();
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         const auto t1 = std::chrono::steady_clock::now();
         best = std::min(best, std::chrono::duration<double>(t1 - t0).count());
     }
@@ -124,7 +132,7 @@ int main(int argc, char** argv)
         return 1;
     }
     std::vector<double> out(nin);
-    hipMemcpy(out.data(), d_out, nin * sizeof(double), hipMemcpyDeviceToHost);
+    hip_check(hipMemcpy(out.data(), d_out, nin * sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
     double ss = 0.0;
     for (const double v : out) {
         ss += v * v;
@@ -134,7 +142,7 @@ int main(int argc, char** argv)
               << " GB/s = " << 1.0e-9 * double(bytes) / best << "\n";
     std::cout << "norm = " << std::sqrt(ss) << "\n";
     for (double* d : {d_in, d_q, d_b, d_d, d_out}) {
-        hipFree(d);
+        (void)hipFree(d);
     }
     return 0;
 }
