@@ -277,18 +277,20 @@ def mixed_batch(steps, warmup, nelmt=200000):
     torch.manual_seed(99)
     u = torch.rand(plan0.ndof, dtype=torch.float64, device=dev) * 2 - 1
     v = torch.empty_like(u)
-    div, _ = mixed.tune(order, bases, G, u, v, steps=5)
-    plan = mixed.MixedPlan(order, div_by_order=div)
-    nbytes = 8 * (2 * plan.ndof + plan.ngeo)
-    out = {"nelmt": nelmt, "counts_by_p": np.bincount(order, minlength=9)[1:].tolist(),
-           "div_by_p": div[1:], "ntiles": plan.ntiles, "bytes": nbytes}
-    for name, by_order in (("single_launch_graph", False), ("per_order_graph", True)):
+    div1, _ = mixed.tune_single(order, bases, G, u, v, steps=5)
+    divp, _ = mixed.tune(order, bases, G, u, v, steps=5)
+    nbytes = 8 * (2 * plan0.ndof + plan0.ngeo)
+    out = {"nelmt": nelmt, "counts_by_p": np.bincount(order, minlength=9)[1:].tolist(), "bytes": nbytes}
+    for name, by_order, div in (("single_launch_graph", False, div1), ("per_order_graph", True, divp)):
+        plan = mixed.MixedPlan(order, div_by_order=div)
         g = plan.graph(bases, G, u, v, by_order=by_order)
         ms, _ = time_steps(g.launch, steps, warmup)
         g.close()
         out[name] = {"ms": round(ms, 4), "GDoF/s": round(plan.ndof / (ms * 1e-3) * 1e-9, 3),
-                     "GB/s": round(nbytes / (ms * 1e-3) * 1e-9, 1)}
-    plan.close()
+                     "GB/s": round(nbytes / (ms * 1e-3) * 1e-9, 1), "div_by_p": div[1:]}
+        if not by_order:
+            out[name]["ntiles"] = plan.ntiles
+        plan.close()
     plan0.close()
     return out
 

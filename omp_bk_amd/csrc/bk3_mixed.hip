@@ -9,14 +9,15 @@
 //   G:     element e's 6 q_e^3 factors (factor-major, as BK3) at
 //          goff[e] = sum_{e' < e} 6 q_e'^3
 //   bases: for p = 1..8 in turn, B_p (n x q) then D_p (q x q)
-// The plan (host side) sorts the elements by order, cuts each order's list
-// into tiles of rounds x epb elements (epb = elements per workgroup of that
-// order's kernel body, chosen per order by the tuner) and keeps two tile
-// lists on the device: grouped by order (for one launch per order) and
-// sorted by decreasing cost (for the persistent launch, where workgroup g
-// takes tiles g, g + grid, ...: a static longest-first assignment, no
-// atomics, no counters to reset).  Every tile runs the transposing BK3 body
-// of bk13t_body.hpp for its order.
+// The plan (host side) sorts the elements by order and keeps each element's
+// offsets at its position in the sorted list.  The single launch cuts each
+// order's part of the list into tiles of rounds x epb elements (epb =
+// elements per workgroup of that order's kernel body, chosen per order by
+// the tuner), sorted by decreasing cost (persistent: workgroup g takes tiles
+// g, g + grid, ...: a static longest-first assignment, no atomics, no
+// counters to reset).  The per-order launches run each order's part of the
+// list with a kernel compiled for that order alone.  Both run the
+// transposing BK3 body of bk13t_body.hpp.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -65,8 +66,8 @@ constexpr int max_smem()
 constexpr int kMixedSmem = max_smem<8>();   // doubles
 
 template <int P, int DIV, bool LOOP>
-__device__ __forceinline__ void run_tile(int tid, const MixedTile& t, const int32_t* __restrict__ elems,
-                                         const int64_t* __restrict__ uoff, const int64_t* __restrict__ goff,
+__device__ __forceinline__ void run_tile(int tid, const MixedTile& t, const int64_t* __restrict__ uoff,
+                                         const int64_t* __restrict__ goff,
                                          const double* __restrict__ bases, const int64_t* __restrict__ boff,
                                          const double* __restrict__ G, const double* __restrict__ u,
                                          double* __restrict__ v, double* smem)
@@ -81,14 +82,14 @@ __device__ __forceinline__ void run_tile(int tid, const MixedTile& t, const int3
     if constexpr (!LOOP) {
         // one round: the tile holds at most epb elements
         const bool active = slot < S::epb && slot < t.count;
-        const int e = elems[t.first + (active ? slot : 0)];
+        const int e = t.first + (active ? slot : 0);
         bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf>(B, D, G + goff[e], u + uoff[e],
                                                                             v + uoff[e], active, slot, l, smem);
     } else {
         for (int r0 = 0; r0 < t.count; r0 += S::epb) {
             const int idx = r0 + slot;
             const bool active = slot < S::epb && idx < t.count;
-            const int e = elems[t.first + (active ? idx : 0)];
+            const int e = t.first + (active ? idx : 0);
             bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf>(
                 B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l, smem, t.first + r0);
             __syncthreads();
@@ -97,34 +98,34 @@ __device__ __forceinline__ void run_tile(int tid, const MixedTile& t, const int3
 }
 
 template <int DIV, bool LOOP>
-__device__ __forceinline__ void run_tile_p(int tid, const MixedTile& t, const int32_t* elems, const int64_t* uoff,
-                                           const int64_t* goff, const double* bases, const int64_t* boff,
+__device__ __forceinline__ void run_tile_p(int tid, const MixedTile& t, const int64_t* uoff, const int64_t* goff,
+                                           const double* bases, const int64_t* boff,
                                            const double* G, const double* u, double* v, double* smem)
 {
     switch (t.p) {
     case 1:
-        run_tile<1, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<1, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 2:
-        run_tile<2, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<2, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 3:
-        run_tile<3, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<3, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 4:
-        run_tile<4, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<4, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 5:
-        run_tile<5, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<5, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 6:
-        run_tile<6, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<6, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 7:
-        run_tile<7, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<7, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     default:
-        run_tile<8, DIV, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<8, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     }
 }
@@ -137,8 +138,8 @@ __device__ __forceinline__ void run_tile_p(int tid, const MixedTile& t, const in
 // do not become scalar loads), which costs registers and occupancy.
 template <bool LOOP>
 __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
-    int ntiles, const MixedTile* __restrict__ tiles, const int32_t* __restrict__ elems,
-    const int64_t* __restrict__ uoff, const int64_t* __restrict__ goff, const double* __restrict__ bases,
+    int ntiles, const MixedTile* __restrict__ tiles, const int64_t* __restrict__ uoff,
+    const int64_t* __restrict__ goff, const double* __restrict__ bases,
     const int64_t* __restrict__ boff, const double* __restrict__ G, const double* __restrict__ u,
     double* __restrict__ v)
 {
@@ -146,9 +147,9 @@ __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
     auto one = [&](int k, int tid) {
         const MixedTile t = tiles[k];
         if (t.div == 2) {
-            run_tile_p<2, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+            run_tile_p<2, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         } else {
-            run_tile_p<1, LOOP>(tid, t, elems, uoff, goff, bases, boff, G, u, v, smem);
+            run_tile_p<1, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         }
     };
     if constexpr (LOOP) {
@@ -167,6 +168,36 @@ __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
     }
 }
 
+// One order's tiles, with that order's body alone: registers and LDS sized
+// for P (the all-order kernel above is sized for its largest body, which
+// costs the low orders occupancy), u and v streamed with non-temporal
+// accesses and three rows of G in flight, as the fastest single-order BK3
+// variants (with their register budget: at least four waves per SIMD).  Workgroup k runs tile k of the order's range, looping when the
+// tile holds more than one workgroup's elements.  The arithmetic is the
+// all-order kernel's: the two give the same bits.
+constexpr int kOrderGpf = 3;
+constexpr int kOrderNt = 1;
+
+template <int P, int DIV>
+__global__ __launch_bounds__((bk13t::Bk13tShape<P + 1, P + 2, DIV>::threads))
+__attribute__((amdgpu_waves_per_eu(4))) void bk3_order_kernel(
+    int first, int count, const int64_t* __restrict__ uoff, const int64_t* __restrict__ goff,
+    const double* __restrict__ B, const double* __restrict__ D, const double* __restrict__ G,
+    const double* __restrict__ u, double* __restrict__ v)
+{
+    constexpr int NM = P + 1;
+    constexpr int NQ = P + 2;
+    using S = bk13t::Bk13tShape<NM, NQ, DIV>;
+    __shared__ double smem[S::epb * kMixedBoxes * S::box];
+    const int slot = threadIdx.x / S::nq2;
+    const int l = threadIdx.x - slot * S::nq2;
+    const int idx = int(blockIdx.x) * S::epb + slot;
+    const bool active = slot < S::epb && idx < count;
+    const int e = first + (active ? idx : 0);
+    bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kOrderGpf, BKX_FORM_F64, double, kOrderNt>(
+        B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l, smem);
+}
+
 int n_of(int p) { return p + 1; }
 int q_of(int p) { return p + 2; }
 
@@ -180,12 +211,12 @@ struct bkx_mixed_plan {
     int ntiles = 0;
     int grid = 0;
     bool persistent = false;    // loop over tiles (grid resident workgroups) or one workgroup per tile
-    int tile_begin[10] = {};            // per order: range of the grouped tile list
+    int elem_begin[10] = {};            // per order: range of the order-sorted element list
+    int div_p[10] = {};                 // per order: elements-per-workgroup divisor
+    int64_t boff_h[10] = {};            // per order: offset of B_p in the bases array
     std::vector<int64_t> uoff_h, goff_h;
-    bkx::MixedTile* tiles_grouped = nullptr;
     bkx::MixedTile* tiles_persist = nullptr;
-    int32_t* elems = nullptr;
-    int64_t* uoff = nullptr;
+    int64_t* uoff = nullptr;    // by position in the order-sorted element list
     int64_t* goff = nullptr;
     int64_t* boff = nullptr;
 };
@@ -218,12 +249,46 @@ int launch_tiles(const bkx::MixedTile* tiles, int ntiles, bool persistent, const
     }
     if (persistent) {
         hipLaunchKernelGGL(bkx::bk3_mixed_kernel<true>, dim3(std::min(pl->grid, ntiles)), dim3(bkx::kMixedThreads), 0,
-                           s, ntiles, tiles, pl->elems, pl->uoff, pl->goff, bases, pl->boff, G, in, out);
+                           s, ntiles, tiles, pl->uoff, pl->goff, bases, pl->boff, G, in, out);
     } else {
         hipLaunchKernelGGL(bkx::bk3_mixed_kernel<false>, dim3(ntiles), dim3(bkx::kMixedThreads), 0, s, ntiles, tiles,
-                           pl->elems, pl->uoff, pl->goff, bases, pl->boff, G, in, out);
+                           pl->uoff, pl->goff, bases, pl->boff, G, in, out);
     }
     return bkx::launch_status();
+}
+
+template <int P, int DIV>
+void launch_order_div(int first, int count, const bkx_mixed_plan* pl, const double* bases, const double* G,
+                      const double* in, double* out, hipStream_t s)
+{
+    using S = bkx::bk13t::Bk13tShape<P + 1, P + 2, DIV>;
+    const double* B = bases + pl->boff_h[P];
+    const double* D = B + (P + 1) * (P + 2);
+    const unsigned grid = unsigned((count + S::epb - 1) / S::epb);
+    hipLaunchKernelGGL((bkx::bk3_order_kernel<P, DIV>), dim3(grid), dim3(S::threads), 0, s, first, count, pl->uoff,
+                       pl->goff, B, D, G, in, out);
+}
+
+// One order's elements, by that order's own kernel: workgroup k takes
+// elements k epb .. k epb + epb - 1 of the order's part of the sorted list.
+int launch_order(int p, const bkx_mixed_plan* pl, const double* bases, const double* G, const double* in,
+                 double* out, hipStream_t s)
+{
+    const int first = pl->elem_begin[p];
+    const int count = pl->elem_begin[p + 1] - first;
+    if (count == 0) {
+        return BKX_SUCCESS;
+    }
+    const bool half = pl->div_p[p] == 2;
+    return bkx::with_order<1, 8>(p, [&](auto P) {
+        constexpr int PP = decltype(P)::value;
+        if (half) {
+            launch_order_div<PP, 2>(first, count, pl, bases, G, in, out, s);
+        } else {
+            launch_order_div<PP, 1>(first, count, pl, bases, G, in, out, s);
+        }
+        return bkx::launch_status();
+    });
 }
 
 }  // namespace
@@ -277,8 +342,9 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
             pl->persistent = true;
         }
         const int per_tile = epb * rounds;
-        pl->tile_begin[p] = int(grouped.size());
+        pl->div_p[p] = div;
         const int first = int(elems.size());
+        pl->elem_begin[p] = first;
         elems.insert(elems.end(), by_p[p].begin(), by_p[p].end());
         const int cnt = int(by_p[p].size());
         for (int t = 0; t < cnt; t += per_tile) {
@@ -288,7 +354,14 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
             cost.push_back(double(c) * (2.0 * n * n * n + 6.0 * q * q * q));
         }
     }
-    pl->tile_begin[9] = int(grouped.size());
+    pl->elem_begin[9] = int(elems.size());
+    // each element's offsets at its position in the sorted list: a workgroup
+    // finds its elements' data with one load, not through the element index
+    std::vector<int64_t> uoff_s(elems.size()), goff_s(elems.size());
+    for (size_t i = 0; i < elems.size(); ++i) {
+        uoff_s[i] = pl->uoff_h[size_t(elems[i])];
+        goff_s[i] = pl->goff_h[size_t(elems[i])];
+    }
     pl->ntiles = int(grouped.size());
     std::vector<int> idx(grouped.size());
     std::iota(idx.begin(), idx.end(), 0);
@@ -304,6 +377,7 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
         bo += int64_t(bkx::n_of(p)) * bkx::q_of(p) + int64_t(bkx::q_of(p)) * bkx::q_of(p);
     }
     boff[9] = bo;
+    std::copy(boff, boff + 10, pl->boff_h);
 
     if (grid <= 0) {
         int dev = 0, cus = 256, per_cu = 1;
@@ -324,11 +398,9 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
     pl->grid = std::max(1, std::min(grid, std::max(1, pl->ntiles)));
 
     int st = BKX_SUCCESS;
-    st = st ? st : upload(&pl->tiles_grouped, grouped.data(), grouped.size());
     st = st ? st : upload(&pl->tiles_persist, persist.data(), persist.size());
-    st = st ? st : upload(&pl->elems, elems.data(), elems.size());
-    st = st ? st : upload(&pl->uoff, pl->uoff_h.data(), nelmt);
-    st = st ? st : upload(&pl->goff, pl->goff_h.data(), nelmt);
+    st = st ? st : upload(&pl->uoff, uoff_s.data(), nelmt);
+    st = st ? st : upload(&pl->goff, goff_s.data(), nelmt);
     st = st ? st : upload(&pl->boff, boff, 10);
     if (st != BKX_SUCCESS) {
         bkx_mixed_plan_destroy(pl);
@@ -343,8 +415,8 @@ int bkx_mixed_plan_destroy(bkx_mixed_plan* plan)
     if (!plan) {
         return BKX_SUCCESS;
     }
-    for (void* ptr : {static_cast<void*>(plan->tiles_grouped), static_cast<void*>(plan->tiles_persist),
-                      static_cast<void*>(plan->elems), static_cast<void*>(plan->uoff),
+    for (void* ptr : {static_cast<void*>(plan->tiles_persist),
+                      static_cast<void*>(plan->uoff),
                       static_cast<void*>(plan->goff), static_cast<void*>(plan->boff)}) {
         if (ptr) {
             (void)hipFree(ptr);
@@ -413,9 +485,7 @@ int bkx_bk3_mixed_apply_by_order_f64(const bkx_mixed_plan* plan, const double* b
         return BKX_ERROR_NULL;
     }
     for (int p = 1; p <= 8; ++p) {
-        const int n = plan->tile_begin[p + 1] - plan->tile_begin[p];
-        const int st = launch_tiles(plan->tiles_grouped + plan->tile_begin[p], n, false, plan, bases, G, in, out,
-                                    reinterpret_cast<hipStream_t>(stream));
+        const int st = launch_order(p, plan, bases, G, in, out, reinterpret_cast<hipStream_t>(stream));
         if (st != BKX_SUCCESS) {
             return st;
         }
@@ -435,9 +505,7 @@ int bkx_bk3_mixed_apply_order_f64(const bkx_mixed_plan* plan, int p, const doubl
     if (!bases || !G || !in || !out) {
         return BKX_ERROR_NULL;
     }
-    const int n = plan->tile_begin[p + 1] - plan->tile_begin[p];
-    return launch_tiles(plan->tiles_grouped + plan->tile_begin[p], n, false, plan, bases, G, in, out,
-                        reinterpret_cast<hipStream_t>(stream));
+    return launch_order(p, plan, bases, G, in, out, reinterpret_cast<hipStream_t>(stream));
 }
 
 int bkx_bk3_mixed_graph_create(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,

@@ -8,10 +8,11 @@ shuffled list of the eight orders, each element with its own
 perturbed-affine geometry (x = x0 + A_e xi, A_e = h (I + a R_e), R_e uniform
 in [-1, 1]^{3x3}) and FP64 uniform[-1, 1] inputs.
 
-The plan cuts each order's elements into tiles; ``tune`` picks, per order,
-the elements per workgroup (divisor 1 or 2 of floor(256 / q^2)) by timing
-that order's launch, then the batch runs as one launch over all tiles
-(optionally captured in a hipGraph) or, for comparison, one launch per order.
+The batch runs as one launch over all orders or as one launch per order
+(each order by a kernel compiled for it alone), either captured in a
+hipGraph.  The elements per workgroup (divisor 1 or 2 of floor(256 / q^2))
+are tuned per order for each: ``tune_single`` for the single launch,
+``tune`` for the per-order launches.
 """
 import ctypes
 import statistics
@@ -229,8 +230,10 @@ def _median_ms(fn, steps=10, warmup=2):
 
 
 def tune(order, bases, G, u, v, steps=10):
-    """Per order, the elements-per-workgroup divisor (1 or 2) whose launch of
-    that order's tiles is fastest; returns (div_by_order, {p: {div: ms}})."""
+    """For the per-order launches: per order, the elements-per-workgroup
+    divisor (1 or 2) whose launch of that order's elements (by the kernel
+    compiled for that order) is fastest; returns (div_by_order,
+    {p: {div: ms}})."""
     plans = {d: MixedPlan(order, div_by_order=[d] * 9) for d in (1, 2)}
     best = [1] * 9
     times = {}
@@ -242,3 +245,28 @@ def tune(order, bases, G, u, v, steps=10):
     for pl in plans.values():
         pl.close()
     return best, times
+
+
+def tune_single(order, bases, G, u, v, steps=10):
+    """For the single launch over all orders, whose kernel is sized for its
+    largest order body: the divisors per order, chosen greedily over the
+    orders present, most elements first, each set to 2 when the whole launch
+    gets faster; returns (div_by_order, ms of the chosen plan)."""
+    counts = np.bincount(np.asarray(order), minlength=9)
+
+    def timed(div):
+        pl = MixedPlan(order, div_by_order=div)
+        try:
+            return _median_ms(lambda: pl.apply(bases, G, u, v), steps)
+        finally:
+            pl.close()
+
+    best = [1] * 9
+    best_ms = timed(best)
+    for p in sorted((p for p in range(1, 9) if counts[p] > 0), key=lambda p: -counts[p]):
+        trial = list(best)
+        trial[p] = 2
+        ms = timed(trial)
+        if ms < best_ms:
+            best, best_ms = trial, ms
+    return best, best_ms

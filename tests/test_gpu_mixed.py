@@ -81,6 +81,19 @@ def test_tuned_divisors():
     _oracle_check(order, tuned, bases, G, u, vt.cpu().numpy())
 
 
+def test_tuned_single_launch_divisors():
+    order, plan, bases, G, u = _batch(3000, seed=6)
+    b, g, uu = _t(bases), _t(G), _t(u)
+    v = torch.empty_like(uu)
+    div, ms = mixed.tune_single(order, b, g, uu, v, steps=3)
+    assert len(div) == 9 and set(div) <= {1, 2} and ms > 0
+    tuned = mixed.MixedPlan(order, div_by_order=div)
+    vt = torch.full_like(uu, float("nan"))
+    tuned.apply(b, g, uu, vt)
+    torch.cuda.synchronize()
+    _oracle_check(order, tuned, bases, G, u, vt.cpu().numpy())
+
+
 def test_graph_replays():
     """Each replay of the captured launch, with eager work between replays,
     equals the eager result."""
