@@ -378,8 +378,26 @@ def cpu_baseline(w, sample_elems=4096, budget_s=10.0):
         best = min(best, time.perf_counter() - t0)
         reps += 1
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": round(ne * n ** 3 / best * 1e-9, 4), "unit": "GDoF/s", "cores": threads, "kind": kind,
-            "sample": "BK5 p=%d FP64, first %d of the %d elements, min of %d applies" % (p, ne, E, reps)}
+    out = {"value": round(ne * n ** 3 / best * 1e-9, 4), "unit": "GDoF/s", "cores": threads, "kind": kind,
+           "sample": "BK5 p=%d FP64, first %d of the %d elements, min of %d applies" % (p, ne, E, reps)}
+    # the same kernel on one thread (its OpenMP loop through libgomp's
+    # thread-count setting for the calling thread), min of 3 applies
+    try:
+        gomp = ctypes.CDLL("libgomp.so.1")
+    except OSError:
+        gomp = None
+    if gomp is not None:
+        gomp.omp_set_num_threads(1)
+        try:
+            one = float("inf")
+            for _ in range(3):
+                t0 = time.perf_counter()
+                call()
+                one = min(one, time.perf_counter() - t0)
+        finally:
+            gomp.omp_set_num_threads(threads)
+        out["value_1thread"] = round(ne * n ** 3 / one * 1e-9, 4)
+    return out
 
 
 def traffic(name):

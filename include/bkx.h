@@ -275,10 +275,10 @@ int bkx_mixed_plan_offsets(const bkx_mixed_plan* plan, int64_t* uoff, int64_t* g
  * made with rounds_by_order or a grid > 0. */
 int bkx_bk3_mixed_apply_f64(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
                             double* out, bkx_stream_t stream);
-/* One launch per order (the comparison). */
+/* One launch per order, each by a kernel compiled for that order alone. */
 int bkx_bk3_mixed_apply_by_order_f64(const bkx_mixed_plan* plan, const double* bases, const double* G,
                                      const double* in, double* out, bkx_stream_t stream);
-/* The launch of one order's tiles only (used by the tuner). */
+/* The launch of one order's elements only (used by the tuner). */
 int bkx_bk3_mixed_apply_order_f64(const bkx_mixed_plan* plan, int p, const double* bases, const double* G,
                                   const double* in, double* out, bkx_stream_t stream);
 /* The persistent launch (by_order = 0) or the per-order launches captured
