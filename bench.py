@@ -179,8 +179,10 @@ def geometry_lines(steps, warmup, p5=7, p3=7, E=65536):
 
 def assembled_bk5(steps, warmup, p=7, E=65536):
     """The assembled BK5 operator v_L = R^T A R u_L (continuous GLL space) on
-    the 64 x 32 x 32 perturbed mesh, p = 7: gather, the element operator and
-    the deterministic scatter, timed together and apart.  Rates per global
+    the 64 x 32 x 32 perturbed mesh, p = 7: the element operator with the
+    gather fused into its loads, then the deterministic scatter, timed
+    together and apart, and beside them the unfused gather, operator and
+    scatter.  Rates per global
     (L-vector) DoF and per element (E-vector) DoF."""
     import torch
     from omp_bk_amd import geometry as geo
@@ -197,18 +199,25 @@ def assembled_bk5(steps, warmup, p=7, E=65536):
     vE = torch.empty_like(uE)
     vL = torch.empty_like(uL)
 
-    def apply():
+    def apply():                       # gather fused into BK5's loads of u
+        R.bk5(p, D, G, uL, out=vE)
+        R.scatter(vE, out=vL)
+
+    def apply_unfused():
         R.gather(uL, out=uE)
         ops.bk5(p, D, G, uE, out=vE)
         R.scatter(vE, out=vL)
     ms, _ = time_steps(apply, steps, warmup)
+    ms_u, _ = time_steps(apply_unfused, steps, warmup)
+    ms_f, _ = time_steps(lambda: R.bk5(p, D, G, uL, out=vE), steps, warmup)
     ms_g, _ = time_steps(lambda: R.gather(uL, out=uE), steps, warmup)
     ms_s, _ = time_steps(lambda: R.scatter(vE, out=vL), steps, warmup)
     R.close()
     return {"bk5_p7_assembled": {"value": round(lsize / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s (global)",
                                  "edofs_per_s": round(E * n ** 3 / (ms * 1e-3) * 1e-9, 3), "ms_per_step": round(ms, 5),
-                                 "global_dofs": lsize, "element_dofs": E * n ** 3, "gather_ms": round(ms_g, 5),
-                                 "scatter_ms": round(ms_s, 5)}}
+                                 "global_dofs": lsize, "element_dofs": E * n ** 3,
+                                 "restricted_bk5_ms": round(ms_f, 5), "scatter_ms": round(ms_s, 5),
+                                 "unfused_ms": round(ms_u, 5), "gather_ms": round(ms_g, 5)}}
 
 
 def rate(w, ms):

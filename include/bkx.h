@@ -242,6 +242,13 @@ int bkx_restrict_gather_f64(const bkx_restrict* plan, const double* uL, double* 
 /* vL[g] = sum of vE[x] over offsets[x] = g, x ascending: deterministic, and
  * equal bit for bit to the sequential accumulation (vL is overwritten). */
 int bkx_restrict_scatter_f64(const bkx_restrict* plan, const double* vE, double* vL, bkx_stream_t stream);
+/* BK5 of the restricted L-vector, vE = A_E (R uL), with the gather fused
+ * into the operator's loads of u (no E-vector of u is written): the same
+ * bits as bkx_restrict_gather_f64 followed by bkx_bk5_apply_f64.  The plan's
+ * elements are (p + 1)^3 GLL nodes (else BKX_ERROR_ARG); G and vE as for
+ * bkx_bk5_apply_f64 over its nelmt elements.  No reference counterpart. */
+int bkx_bk5_apply_restricted_f64(const bkx_restrict* plan, int p, const double* dbasis, const double* G,
+                                 const double* uL, double* vE, bkx_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* Mixed-order BK3 batches (no reference counterpart: the reference runs    */

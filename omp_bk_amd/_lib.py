@@ -41,6 +41,7 @@ _SIGNATURES = {
     "bkx_restrict_info": [_vp, _vp, _vp, _vp],
     "bkx_restrict_gather_f64": [_vp, _vp, _vp, _vp],
     "bkx_restrict_scatter_f64": [_vp, _vp, _vp, _vp],
+    "bkx_bk5_apply_restricted_f64": [_vp, _c_int, _vp, _vp, _vp, _vp, _vp],
     "bkx_probe_copy": [_vp, _vp, _c_size, _vp],
     "bkx_probe_read_mix": [_c_int, _vp, _vp, _c_size, _vp],
     "bkx_tune_count": [_c_int, _c_int],

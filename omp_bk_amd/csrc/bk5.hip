@@ -260,9 +260,12 @@ __global__ __launch_bounds__(kWave * WAVES) void bk5_kernel(size_t nelmt, const 
 // 3 nq^3 writes and 4 nq^4 reads when the j and k contractions read their
 // operands from LDS (the slab mapping above).  MINW is the occupancy
 // (wavefronts per SIMD) the register allocation is held to.
-template <int NQ, int WAVES, typename TS, bool NT, int MINW, int GSTAGE, int PD, bool SEQ>
+// GATHER: u is an L-vector read through the element restriction, u(e, x) =
+// u[uidx[e nq^3 + x]] (the gather of restrict.hip fused into the loads).
+template <int NQ, int WAVES, typename TS, bool NT, int MINW, int GSTAGE, int PD, bool SEQ, bool GATHER>
 __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(MINW))) void bk5t_kernel(size_t nelmt, const double* __restrict__ D,
                                                             const TS* __restrict__ G, const TS* __restrict__ u,
+                                                            const int32_t* __restrict__ uidx,
                                                             double* __restrict__ v)
 {
     using S = Bk5Shape<NQ>;
@@ -304,9 +307,23 @@ __global__ __launch_bounds__(kWave * WAVES) __attribute__((amdgpu_waves_per_eu(M
     // unconditional loads (ee is a real element), masked after the
     // conversion: converting inside the condition waits for each load
     TS xraw[NQ];
+    if constexpr (GATHER) {
+        // the indices are read once: streamed
+        const int32_t* ie = uidx + ee * nq3 + a * NQ + b;
+        int32_t g[NQ];
 #pragma unroll
-    for (int c = 0; c < NQ; ++c) {
-        xraw[c] = load<NT>(ue + c * nq2);
+        for (int c = 0; c < NQ; ++c) {
+            g[c] = __builtin_nontemporal_load(ie + c * nq2);
+        }
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            xraw[c] = u[g[c]];
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NQ; ++c) {
+            xraw[c] = load<NT>(ue + c * nq2);
+        }
     }
     double x[NQ];
 #pragma unroll
@@ -816,15 +833,17 @@ int launch_bk5p_v(size_t nelmt, const double* D, const TS* G, const TS* u, doubl
     return launch_status();
 }
 
-template <int NQ, int WAVES, typename TS, bool NT, int MINW = 1, int GSTAGE = 0, int PD = 1, bool SEQ = false>
-int launch_bk5t_v(size_t nelmt, const double* D, const TS* G, const TS* u, double* v, hipStream_t s)
+template <int NQ, int WAVES, typename TS, bool NT, int MINW = 1, int GSTAGE = 0, int PD = 1, bool SEQ = false,
+          bool GATHER = false>
+int launch_bk5t_v(size_t nelmt, const double* D, const TS* G, const TS* u, double* v, hipStream_t s,
+                  const int32_t* uidx = nullptr)
 {
     constexpr size_t per_block = size_t(WAVES) * Bk5Shape<NQ>::epw;
     const unsigned grid = grid_for(nelmt, per_block);
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk5t_kernel<NQ, WAVES, TS, NT, MINW, GSTAGE, PD, SEQ>), dim3(grid), dim3(kWave * WAVES), 0, s, nelmt, D, G, u, v);
+    hipLaunchKernelGGL((bk5t_kernel<NQ, WAVES, TS, NT, MINW, GSTAGE, PD, SEQ, GATHER>), dim3(grid), dim3(kWave * WAVES), 0, s, nelmt, D, G, u, uidx, v);
     return launch_status();
 }
 
@@ -981,9 +1000,48 @@ int launch_bk5(int variant, size_t nelmt, const double* D, const TS* G, const TS
     }
 }
 
+// BK5 with the gather fused into the loads of u (GATHER): the launch
+// shapes of the variants that are defaults at some order (18, 27, 34-36,
+// 38); any other variant runs as 36.  Variants 34-38 need a 16-byte-aligned
+// G, else 17's shape.
+template <int NQ>
+int launch_bk5_gather(int variant, size_t nelmt, const double* D, const double* G, const int32_t* uidx,
+                      const double* uL, double* v, hipStream_t s)
+{
+    const bool g16 = reinterpret_cast<uintptr_t>(G) % 16 == 0;
+    if (!g16 && variant >= 34) {
+        return launch_bk5t_v<NQ, 1, double, false, 1, true, 8, false, true>(nelmt, D, G, uL, v, s, uidx);
+    }
+    switch (variant) {
+    case 18:
+        return launch_bk5t_v<NQ, 2, double, false, 1, true, 4, false, true>(nelmt, D, G, uL, v, s, uidx);
+    case 27:
+        return launch_bk5t_v<NQ, 2, double, false, 4, true, 4, true, true>(nelmt, D, G, uL, v, s, uidx);
+    case 34:
+        return launch_bk5t_v<NQ, 1, double, false, 1, 2, 8, false, true>(nelmt, D, G, uL, v, s, uidx);
+    case 35:
+        return launch_bk5t_v<NQ, 1, double, false, 1, 2, 4, false, true>(nelmt, D, G, uL, v, s, uidx);
+    case 38:
+        return launch_bk5t_v<NQ, 2, double, false, 1, 2, 4, false, true>(nelmt, D, G, uL, v, s, uidx);
+    default:
+        if (!g16) {
+            return launch_bk5t_v<NQ, 1, double, false, 1, true, 8, false, true>(nelmt, D, G, uL, v, s, uidx);
+        }
+        return launch_bk5t_v<NQ, 1, double, false, 1, 2, 2, false, true>(nelmt, D, G, uL, v, s, uidx);
+    }
+}
+
 }  // namespace
 
 int bk5_variants() { return kBk5Variants; }
+
+int bk5_apply_gather_f64(int variant, int p, size_t nelmt, const double* D, const double* G, const int32_t* uidx,
+                         const double* uL, double* v, hipStream_t s)
+{
+    return with_order<1, 7>(p, [&](auto P) {
+        return launch_bk5_gather<decltype(P)::value + 1>(variant, nelmt, D, G, uidx, uL, v, s);
+    });
+}
 
 int bk5_apply_f64(int variant, int p, size_t nelmt, const double* D, const double* G, const double* u, double* v,
                   hipStream_t s)

@@ -2,7 +2,8 @@
 // the L-vector (one value per global node, continuous across element
 // faces) to the E-vector the operators read (gather, u_E = R u_L) and back
 // (scatter-add, v_L = R^T v_E), so that R^T A_E R is the assembled operator
-// of a continuous Galerkin discretisation.  No reference counterpart: the
+// of a continuous Galerkin discretisation.  For BK5 the gather can also run
+// inside the operator's loads (bkx_bk5_apply_restricted_f64).  No reference counterpart: the
 // reference applies its operators to E-vectors only.
 //
 // offsets[x], x = e * elemsize + t, is the global node of element e's local
@@ -32,6 +33,10 @@ struct bkx_restrict {
 };
 
 namespace bkx {
+
+int bk5_apply_gather_f64(int variant, int p, size_t nelmt, const double* D, const double* G, const int32_t* uidx,
+                         const double* uL, double* v, hipStream_t s);   // bk5.hip
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -193,6 +198,28 @@ int bkx_restrict_scatter_f64(const bkx_restrict* plan, const double* vE, double*
     hipLaunchKernelGGL(bkx::scatter_kernel, dim3(grid), dim3(bkx::kThreads), 0,
                        reinterpret_cast<hipStream_t>(stream), plan->lsize, plan->row, plan->col, vE, vL);
     return bkx::launch_status();
+}
+
+int bkx_bk5_apply_restricted_f64(const bkx_restrict* plan, int p, const double* dbasis, const double* G,
+                                 const double* uL, double* vE, bkx_stream_t stream)
+{
+    if (!plan) {
+        return BKX_ERROR_NULL;
+    }
+    if (p < bkx_min_order(5) || p > bkx_max_order(5)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (plan->elemsize != (p + 1) * (p + 1) * (p + 1)) {
+        return BKX_ERROR_ARG;
+    }
+    if (plan->nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!dbasis || !G || !uL || !vE) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk5_apply_gather_f64(bkx_tune_get(5, BKX_FORM_F64, p), p, plan->nelmt, dbasis, G, plan->offsets, uL,
+                                     vE, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

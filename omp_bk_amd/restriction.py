@@ -5,6 +5,7 @@ node) to the operators' E-vectors and back, through the C ABI
     R = Restriction(offsets, lsize)     # offsets: host int32 (E, n^3)
     uE = R.gather(uL);  vL = R.scatter(vE)
     vL = R.scatter(ops.bk5(p, D, G, R.gather(uL)))
+    vL = R.scatter(R.bk5(p, D, G, uL))  # the same, gather fused into BK5
 
 No reference counterpart: the reference applies its operators to E-vectors
 only.  The scatter is deterministic (each node's contributions summed in
@@ -65,4 +66,23 @@ class Restriction:
         with torch.cuda.device(vE.device):
             _lib.call("bkx_restrict_scatter_f64", self._plan, vE.data_ptr(), out.data_ptr(),
                       torch.cuda.current_stream(vE.device).cuda_stream)
+        return out
+
+    def bk5(self, p, D, G, uL, out=None):
+        """vE = A_E (R uL): BK5 (FP64) with the gather fused into the
+        operator's loads of u, the same bits as ops.bk5(p, D, G,
+        self.gather(uL)).  Elements of (p + 1)^3 GLL nodes."""
+        n = p + 1
+        if self.elemsize != n ** 3:
+            raise ValueError("the restriction's elements have %d nodes, BK5 at p = %d needs %d"
+                             % (self.elemsize, p, n ** 3))
+        self._check("uL", uL, self.lsize)
+        self._check("D", D, n * n)
+        self._check("G", G, self.nelmt * 6 * n ** 3)
+        if out is None:
+            out = torch.empty(self.nelmt * self.elemsize, dtype=torch.float64, device=uL.device)
+        self._check("out", out, self.nelmt * self.elemsize)
+        with torch.cuda.device(uL.device):
+            _lib.call("bkx_bk5_apply_restricted_f64", self._plan, p, D.data_ptr(), G.data_ptr(), uL.data_ptr(),
+                      out.data_ptr(), torch.cuda.current_stream(uL.device).cuda_stream)
         return out

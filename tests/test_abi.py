@@ -62,6 +62,7 @@ def test_error_paths_without_device():
     assert L.bkx_restrict_create(1, 4, ptr, 9, ctypes.byref(h)) == 5      # offset 9 >= lsize
     assert L.bkx_restrict_gather_f64(None, None, None, None) == 2
     assert L.bkx_restrict_scatter_f64(None, None, None, None) == 2
+    assert L.bkx_bk5_apply_restricted_f64(None, 7, None, None, None, None, None) == 2
 
 
 def test_python_errors():

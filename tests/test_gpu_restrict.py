@@ -88,3 +88,68 @@ def test_assembled_bk5_benchmark_mesh():
     assert abs(float(torch.dot(Au, w).item()) - float(torch.dot(u, Aw).item())) <= 1e-11 * scale * lsize ** 0.5
     assert float(A(torch.ones_like(u)).abs().max().item()) <= 1e-11 * scale
     R.close()
+
+
+def _fused_case(p, shape, seed):
+    E = int(np.prod(shape))
+    off, lsize = geo.box_offsets(p, shape)
+    rng = np.random.default_rng(seed)
+    D = _t(geo.bk5_basis(p))
+    G = _t(geo.geometric_factors(5, p, E, shape=shape).reshape(-1))
+    uL = _t(rng.uniform(-1, 1, lsize))
+    return off, lsize, D, G, uL
+
+
+@pytest.mark.parametrize("p", range(1, 8))
+def test_restricted_bk5_equals_gather_then_bk5(p):
+    """The gather fused into BK5's loads gives the bits of the separate
+    gather and operator, at every order (its default launch shape), on a
+    mesh whose element count no workgroup size divides."""
+    off, lsize, D, G, uL = _fused_case(p, (5, 3, 3), p)
+    R = Restriction(off, lsize)
+    ref = ops.bk5(p, D, G, R.gather(uL))
+    got = R.bk5(p, D, G, uL)
+    assert torch.equal(got, ref)
+    R.close()
+
+
+@pytest.mark.parametrize("variant", [18, 27, 34, 35, 36, 38, 0])
+def test_restricted_bk5_launch_shapes(variant):
+    """Each launch shape of the fused path (18, 27, 34-36, 38; others run as
+    36), and the 8-byte-aligned G that 34-38 fall back from (to 17's
+    shape), against the unfused operator of the same shape."""
+    from omp_bk_amd import _lib
+    L = _lib.lib()
+    p = 5
+    off, lsize, D, G, uL = _fused_case(p, (4, 3, 3), 40 + variant)
+    R = Restriction(off, lsize)
+    prev = L.bkx_tune_get(5, 0, p)
+    try:
+        shape = variant if variant in (18, 27, 34, 35, 36, 38) else 36
+        assert L.bkx_tune_set(5, 0, p, shape) == 0
+        ref = ops.bk5(p, D, G, R.gather(uL))
+        Gs = torch.empty(G.numel() + 1, dtype=torch.float64, device=DEV)[1:]   # 8-byte aligned, not 16
+        Gs.copy_(G)
+        ref_s = ops.bk5(p, D, Gs, R.gather(uL))
+        assert L.bkx_tune_set(5, 0, p, variant) == 0
+        assert torch.equal(R.bk5(p, D, G, uL), ref)
+        assert torch.equal(R.bk5(p, D, Gs, uL), ref_s)
+    finally:
+        L.bkx_tune_set(5, 0, p, prev)
+    R.close()
+
+
+def test_restricted_bk5_shuffled_offsets():
+    """Unstructured offsets (random node numbering, repeated nodes)."""
+    p = 3
+    rng = np.random.default_rng(17)
+    E, lsize = 300, 5000
+    off = rng.integers(0, lsize, (E, (p + 1) ** 3)).astype(np.int32)
+    D = _t(geo.bk5_basis(p))
+    G = _t(rng.uniform(0.5, 1.5, E * 6 * (p + 1) ** 3))
+    uL = _t(rng.uniform(-1, 1, lsize))
+    R = Restriction(off, lsize)
+    assert torch.equal(R.bk5(p, D, G, uL), ops.bk5(p, D, G, R.gather(uL)))
+    with pytest.raises(ValueError):
+        R.bk5(4, _t(geo.bk5_basis(4)), G, uL)
+    R.close()
