@@ -356,12 +356,37 @@ def probe_roofs(nbytes=2 << 30, steps=20):
     return out
 
 
+_ONE_CORE = r"""
+import ctypes, sys, time
+import numpy as np
+so, fn, arg0, n, ne = sys.argv[1], sys.argv[2], int(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5])
+f = getattr(ctypes.CDLL(so), fn)
+f.argtypes = [ctypes.c_int, ctypes.c_size_t] + [ctypes.c_void_p] * 4
+rng = np.random.default_rng(0)
+D, G = rng.uniform(-1, 1, n * n), rng.uniform(0.5, 1.5, ne * 6 * n ** 3)
+u = rng.uniform(-1, 1, ne * n ** 3)
+v = np.empty_like(u)
+best = float("inf")
+for _ in range(3):
+    t0 = time.perf_counter()
+    f(arg0, ne, D.ctypes.data, G.ctypes.data, u.ctypes.data, v.ctypes.data)
+    best = min(best, time.perf_counter() - t0)
+print(best)
+"""
+
+
 def cpu_baseline(w, sample_elems=4096, budget_s=10.0):
     """BK5 FP64 on the host: the reference's own SumFactorization<double, 8>
     (oracle/_ref, compiled from the reference sources with g++ -O3 -fopenmp)
     when present, else the oracle's restatement; min time over repeated
-    applies of a sample of the workload's elements."""
+    applies of a sample of the workload's elements.  cores is the number of
+    cores the applies kept busy (process CPU time over wall time: the
+    reference's `target teams loop` runs as host teams, whose count
+    OMP_NUM_THREADS does not set), and value_1thread the rate of the same
+    kernel in a process pinned to one core."""
     import numpy as np
+    import subprocess
+    import sys
     p, E = w["p"], w["E"]
     n = p + 1
     ne = min(sample_elems, E)
@@ -371,41 +396,33 @@ def cpu_baseline(w, sample_elems=4096, budget_s=10.0):
     v = np.empty_like(u)
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libbkref.so")
     port_so = os.path.join(ROOT, "oracle", "liboracle.so")
-    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
     if os.path.exists(ref_so):
-        lib, kind = ctypes.CDLL(ref_so), "reference"
-        call = lambda: lib.bkref_bk5_sumfact(ctypes.c_int(p), ctypes.c_size_t(ne), dp(D), dp(G), dp(u), dp(v))
+        so, kind, fn, arg0 = ref_so, "reference", "bkref_bk5_sumfact", p
     elif os.path.exists(port_so):
-        lib, kind = ctypes.CDLL(port_so), "port"
-        call = lambda: lib.bko_bk5_sumfact(ctypes.c_int(n), ctypes.c_size_t(ne), dp(D), dp(G), dp(u), dp(v))
+        so, kind, fn, arg0 = port_so, "port", "bko_bk5_sumfact", n
     else:
         return None
+    f = getattr(ctypes.CDLL(so), fn)
+    f.argtypes = [ctypes.c_int, ctypes.c_size_t] + [ctypes.c_void_p] * 4
+    call = lambda: f(arg0, ne, D.ctypes.data, G.ctypes.data, u.ctypes.data, v.ctypes.data)
     best, t_start, reps = float("inf"), time.perf_counter(), 0
+    wall0, cpu0 = time.perf_counter(), time.process_time()
     while reps < 3 or (time.perf_counter() - t_start < budget_s and reps < 50):
         t0 = time.perf_counter()
         call()
         best = min(best, time.perf_counter() - t0)
         reps += 1
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    out = {"value": round(ne * n ** 3 / best * 1e-9, 4), "unit": "GDoF/s", "cores": threads, "kind": kind,
-           "sample": "BK5 p=%d FP64, first %d of the %d elements, min of %d applies" % (p, ne, E, reps)}
-    # the same kernel on one thread (its OpenMP loop through libgomp's
-    # thread-count setting for the calling thread), min of 3 applies
+    busy = (time.process_time() - cpu0) / (time.perf_counter() - wall0)
+    out = {"value": round(ne * n ** 3 / best * 1e-9, 4), "unit": "GDoF/s", "cores": max(1, int(round(busy))),
+           "kind": kind, "sample": "BK5 p=%d FP64, first %d of the %d elements, min of %d applies" % (p, ne, E, reps),
+           "cores_available": len(os.sched_getaffinity(0))}
     try:
-        gomp = ctypes.CDLL("libgomp.so.1")
-    except OSError:
-        gomp = None
-    if gomp is not None:
-        gomp.omp_set_num_threads(1)
-        try:
-            one = float("inf")
-            for _ in range(3):
-                t0 = time.perf_counter()
-                call()
-                one = min(one, time.perf_counter() - t0)
-        finally:
-            gomp.omp_set_num_threads(threads)
-        out["value_1thread"] = round(ne * n ** 3 / one * 1e-9, 4)
+        core = min(os.sched_getaffinity(0))
+        r = subprocess.run([sys.executable, "-c", _ONE_CORE, so, fn, str(arg0), str(n), str(ne)], capture_output=True,
+                           text=True, timeout=120, preexec_fn=lambda: os.sched_setaffinity(0, {core}))
+        out["value_1thread"] = round(ne * n ** 3 / float(r.stdout.split()[-1]) * 1e-9, 4)
+    except (OSError, ValueError, IndexError, subprocess.SubprocessError):
+        pass
     return out
 
 
