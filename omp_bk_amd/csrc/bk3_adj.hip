@@ -32,10 +32,18 @@ namespace {
 
 enum AdjMode { kLoss = 0, kVjp = 1, kApply = 2 };
 
+// Elements per workgroup: floor(256 / q^2) divided by adj_div(p).  The
+// kernel holds up to 168 VGPRs and three LDS boxes per element, so at most
+// orders fewer elements per workgroup let more workgroups share a CU; the
+// divisors are the fastest measured on MI355X (the adjoint at 5.6 M DoF, p =
+// 6: one element per workgroup, 31.4 GDoF/s against 25.2 with four).
+constexpr int adj_div(int p) { return (p == 4 || p == 7) ? 1 : (p == 8 ? 2 : 4); }
+
 template <int NM, int NQ>
 struct AdjShape {
     static constexpr int nq2 = NQ * NQ;
-    static constexpr int epb = 256 / nq2 > 0 ? 256 / nq2 : 1;
+    static constexpr int full = 256 / nq2 > 0 ? 256 / nq2 : 1;
+    static constexpr int epb = full / adj_div(NM - 1) > 0 ? full / adj_div(NM - 1) : 1;
     static constexpr int threads = ((epb * nq2 + kWave - 1) / kWave) * kWave;
 };
 
