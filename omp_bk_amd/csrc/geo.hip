@@ -78,7 +78,6 @@ __global__ __launch_bounds__((GeoShape<NM, NQ>::threads)) void geo_gauss_kernel(
     using E = bk3p::Elem<NM, NQ>;
     using S = GeoShape<NM, NQ>;
     __shared__ double s_box[3 * E::box];
-    __shared__ double s_J[9][S::nq3];   // J(c, d) at point (p, lane)
 
     const size_t e = blockIdx.x;
     const int l = threadIdx.x;
@@ -86,7 +85,11 @@ __global__ __launch_bounds__((GeoShape<NM, NQ>::threads)) void geo_gauss_kernel(
     E el;
     el.init(s_box, in_elem ? l : 0, in_elem);
     const bool modal = in_elem && l < E::nm2;
-#pragma unroll 1
+    // J(c, d) at the lane's column of points (p, lane), p < NQ, in registers:
+    // the LDS then holds only the gradient's three boxes, so that several
+    // workgroups share a CU
+    double Jc[3][3][NQ];
+#pragma unroll
     for (int c = 0; c < 3; ++c) {
         double x[NM];
         const double* xc = coords + (e * 3 + c) * E::nm3;
@@ -98,14 +101,12 @@ __global__ __launch_bounds__((GeoShape<NM, NQ>::threads)) void geo_gauss_kernel(
         }
         double qr[NQ];
         el.template grad<0>(Bm, Dm, x, qr);   // ends with the workgroup synchronised
-        if (in_elem) {
 #pragma unroll
-            for (int p = 0; p < NQ; ++p) {
-                const int xx = E::X(p, el.lq, el.lq2);
-                s_J[c * 3 + 0][p * S::nq2 + l] = qr[p];
-                s_J[c * 3 + 1][p * S::nq2 + l] = el.B[xx];
-                s_J[c * 3 + 2][p * S::nq2 + l] = el.C[xx];
-            }
+        for (int p = 0; p < NQ; ++p) {
+            const int xx = E::X(p, el.lq, el.lq2);
+            Jc[c][0][p] = qr[p];
+            Jc[c][1][p] = el.B[xx];
+            Jc[c][2][p] = el.C[xx];
         }
         // the next gradient writes box A, then box B only after a barrier,
         // so this lane's reads of B and C above come first
@@ -114,14 +115,14 @@ __global__ __launch_bounds__((GeoShape<NM, NQ>::threads)) void geo_gauss_kernel(
         return;
     }
     const double wqr = wq[el.lq] * wq[el.lq2];
-#pragma unroll 1
+#pragma unroll
     for (int p = 0; p < NQ; ++p) {
         double J[3][3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                J[c][d] = s_J[c * 3 + d][p * S::nq2 + l];
+                J[c][d] = Jc[c][d][p];
             }
         }
         double g[6], wdet;
