@@ -50,15 +50,41 @@ __global__ __launch_bounds__(kThreads) void gather_kernel(size_t n, const int32_
     }
 }
 
+// Two contributions per step: their positions, then their values, are
+// loaded together (unconditionally, the positions clamped to the node's
+// last one), and the values are added one at a time in ascending order, so
+// that a node's sum takes one dependent pair of loads per two contributions
+// instead of one per contribution and keeps the sequential loop's bits.
+// Measured on the 64 x 32 x 32 box mesh at p = 7: 0.227 ms with one
+// contribution per step, 0.186 with two, 0.191 with four, 0.275 with eight.
+constexpr int kScatterStep = 2;
+
 __global__ __launch_bounds__(kThreads) void scatter_kernel(size_t lsize, const int32_t* __restrict__ row,
                                                            const int32_t* __restrict__ col,
                                                            const double* __restrict__ vE, double* __restrict__ vL)
 {
     const size_t g = size_t(blockIdx.x) * kThreads + threadIdx.x;
     if (g < lsize) {
+        const int32_t t0 = row[g];
+        const int32_t t1 = row[g + 1];
         double s = 0.0;
-        for (int32_t t = row[g]; t < row[g + 1]; ++t) {
-            s += vE[col[t]];
+        for (int32_t t = t0; t < t1; t += kScatterStep) {
+            int32_t c[kScatterStep];
+            double x[kScatterStep];
+#pragma unroll
+            for (int k = 0; k < kScatterStep; ++k) {
+                c[k] = col[t + k < t1 ? t + k : t1 - 1];
+            }
+#pragma unroll
+            for (int k = 0; k < kScatterStep; ++k) {
+                x[k] = vE[c[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < kScatterStep; ++k) {
+                if (t + k < t1) {
+                    s += x[k];
+                }
+            }
         }
         vL[g] = s;
     }
