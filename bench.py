@@ -119,7 +119,37 @@ def bk3_workload(p=7, E=65536):
     u = torch.rand(E * n ** 3, dtype=torch.float64, device=dev) * 2 - 1
     v = torch.empty_like(u)
     fn = lambda: ops.bk3(p, B, D, G, u, out=v)
-    return dict(fn=fn, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + 6 * E * q ** 3), p=p, E=E)
+    return dict(fn=fn, B=B, D=D, G=G, u=u, v=v, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + 6 * E * q ** 3), p=p,
+                E=E)
+
+
+def bk3_reduced(w, steps, warmup, ms64):
+    """BK3 with u and G stored in FP32 (v FP64), f32w (FP64 arithmetic) and
+    f32c (float-float), beside the FP64 line of the same workload; errors as
+    for the BK5 forms."""
+    import torch
+    from omp_bk_amd import ops
+    p, n, q = w["p"], w["p"] + 1, w["p"] + 2
+    G32, u32 = w["G"].float(), w["u"].float()
+    ref = ops.bk3(p, w["B"], w["D"], w["G"], w["u"])
+    ref_q = ops.bk3(p, w["B"], w["D"], G32.double(), u32.double())
+    nb = 4 + 4 * 6 * (q / n) ** 3 + 8
+    out = {}
+    for form in ("f32w", "f32c"):
+        v = torch.empty_like(w["v"])
+        # a longer warm-up: the first applies on the freshly converted
+        # arrays ran up to 10% slower than the rest
+        ms, _ = time_steps(lambda: ops.bk3(p, w["B"], w["D"], G32, u32, out=v, form=form), steps, max(warmup, 20))
+        out["bk3_p%d_%s" % (p, form)] = {
+            "value": round(w["dofs"] / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
+            "speedup_vs_f64": round(ms64 / ms, 3), "bytes_per_dof": round(nb, 3),
+            "frac": round(nb * w["dofs"] / (ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
+            "relerr_vs_f64_same_inputs": float(((v - ref_q).abs().max() / ref_q.abs().max()).item()),
+            "relerr_vs_f64_unrounded_inputs": float(((v - ref).abs().max() / ref.abs().max()).item()),
+        }
+        del v
+    del G32, u32, ref, ref_q
+    return out
 
 
 def bk1_workload(p=3, E=4096):
@@ -496,6 +526,8 @@ def main():
         w = make()
         ms, _ = time_steps(w["fn"], a.steps, a.warmup)
         lines[name] = rate(w, ms)
+        if name == "bk3_p7_f64":
+            lines.update(bk3_reduced(w, a.steps, a.warmup, ms))
         if "out" in w:
             # 0.26 M DoF: the apply is shorter than its launch from the host
             gms, same = graph_ms(w, a.steps, a.warmup)

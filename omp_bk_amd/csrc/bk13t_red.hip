@@ -23,7 +23,7 @@ namespace {
 
 using bk13t::Bk13tShape;
 
-template <int NM, int NQ, bool BK3, int DIV, int NB, int GPF, int FORM>
+template <int NM, int NQ, bool BK3, int DIV, int NB, int GPF, int FORM, int NT>
 __global__ __launch_bounds__((Bk13tShape<NM, NQ, DIV>::threads))
 __attribute__((amdgpu_waves_per_eu(GPF > 0 ? 4 : 1))) void bk13t_red_kernel(
     size_t nelmt, const double* __restrict__ B, const double* __restrict__ D,
@@ -48,12 +48,12 @@ __attribute__((amdgpu_waves_per_eu(GPF > 0 ? 4 : 1))) void bk13t_red_kernel(
         bk13t::bk13t_body<NM, NQ, BK3, DIV, true, NB, GPF, FORM, ff>(
             s_mat, s_mat + NM * NQ, Q + ee * qper, u + ee * S::nm3, v + ee * S::nm3, active, slot, l, s_box);
     } else {
-        bk13t::bk13t_body<NM, NQ, BK3, DIV, false, NB, GPF, FORM, double>(
+        bk13t::bk13t_body<NM, NQ, BK3, DIV, false, NB, GPF, FORM, double, NT>(
             B, D, Q + ee * qper, u + ee * S::nm3, v + ee * S::nm3, active, slot, l, s_box);
     }
 }
 
-template <int NM, bool BK3, int DIV, int NB, int GPF, int FORM>
+template <int NM, bool BK3, int DIV, int NB, int GPF, int FORM, int NT = 0>
 int launch_red(size_t nelmt, const double* B, const double* D, const void* Q, const void* u, double* v,
                hipStream_t s)
 {
@@ -63,19 +63,22 @@ int launch_red(size_t nelmt, const double* B, const double* D, const void* Q, co
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk13t_red_kernel<NM, NM + 1, BK3, DIV, NB, GPF, FORM>), dim3(grid), dim3(S::threads), 0, s,
+    hipLaunchKernelGGL((bk13t_red_kernel<NM, NM + 1, BK3, DIV, NB, GPF, FORM, NT>), dim3(grid), dim3(S::threads), 0, s,
                        nelmt, B, D, static_cast<const TS*>(Q), static_cast<const TS*>(u), v);
     return launch_status();
 }
 
 // Variants per form.  F32W: 0 all elements per workgroup, 1 half (two LDS
 // boxes, BK3 with two rows of G loaded ahead), 2 all (two boxes, BK3 one row
-// ahead in P5), 3 all (three boxes).  F32C and FF: 0 all, 1 half (two
-// boxes), 2 all (three boxes).
+// ahead in P5), 3 all (three boxes); 4-6 are 2, 3 and 0 with non-temporal
+// loads of u and stores of v, 7 is 2 with JxW / G streamed as well, 8 and 9
+// are 6 with three rows of G ahead (all elements or half of them).  F32C
+// and FF (bound by the vector ALU): 0 all, 1 half (two boxes), 2 all (three
+// boxes).
 template <int FORM>
 constexpr int red_variants()
 {
-    return FORM == BKX_FORM_F32W ? 4 : 3;
+    return FORM == BKX_FORM_F32W ? 10 : 3;
 }
 
 template <int NM, bool BK3, int FORM>
@@ -92,6 +95,18 @@ int launch_red_variant(int variant, size_t nelmt, const double* B, const double*
             return launch_red<NM, BK3, 1, 2, 0, FORM>(nelmt, B, D, Q, u, v, s);
         case 3:
             return launch_red<NM, BK3, 1, 3, 0, FORM>(nelmt, B, D, Q, u, v, s);
+        case 4:
+            return launch_red<NM, BK3, 1, 2, 0, FORM, 1>(nelmt, B, D, Q, u, v, s);
+        case 5:
+            return launch_red<NM, BK3, 1, 3, 0, FORM, 1>(nelmt, B, D, Q, u, v, s);
+        case 6:
+            return launch_red<NM, BK3, 1, 2, BK3 ? 2 : 0, FORM, 1>(nelmt, B, D, Q, u, v, s);
+        case 7:
+            return launch_red<NM, BK3, 1, 2, 0, FORM, 3>(nelmt, B, D, Q, u, v, s);
+        case 8:
+            return launch_red<NM, BK3, 1, 2, BK3 ? 3 : 0, FORM, 1>(nelmt, B, D, Q, u, v, s);
+        case 9:
+            return launch_red<NM, BK3, 2, 2, BK3 ? 3 : 0, FORM, 1>(nelmt, B, D, Q, u, v, s);
         default:
             return BKX_ERROR_ARG;
         }

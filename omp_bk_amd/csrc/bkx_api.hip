@@ -57,9 +57,9 @@ hipStream_t as_stream(bkx_stream_t s) { return reinterpret_cast<hipStream_t>(s);
 // Launch variant of each (operator, form, order).  The defaults are the
 // fastest variants measured on an MI355X: the FP64 ones by the variant
 // harnesses (`python -m omp_bk_amd.study pick`: median of three runs at
-// 5e7 DoFs for BK5, 2e7 for BK1 and BK3), the others by omp_bk_amd/tune.py
-// (BK5 over E = 65536, BK1 and BK3 over 32768 elements); the autotuner
-// overwrites them at run time.
+// 5e7 DoFs for BK5, 2e7 for BK1 and BK3), the BK1 and BK3 f32w ones at 2e7
+// DoFs too, the others by omp_bk_amd/tune.py (BK5 over E = 65536, BK1 and
+// BK3 over 32768 elements); the autotuner overwrites them at run time.
 constexpr int kOps = 6;
 constexpr int kForms = 5;
 constexpr int kOrders = 9;
@@ -80,10 +80,10 @@ struct VariantTable {
         static const int bk5_f32w[kOrders] = {0, 10, 16, 35, 27, 39, 22, 34, 0};
         static const int bk3_f64[kOrders] = {0, 16, 14, 16, 17, 16, 13, 16, 18};
         static const int bk1_f64[kOrders] = {0, 5, 1, 19, 18, 19, 18, 19, 18};
-        static const int bk3_f32w[kOrders] = {0, 1, 1, 2, 2, 0, 0, 2, 2};
+        static const int bk3_f32w[kOrders] = {0, 9, 6, 8, 8, 8, 4, 8, 8};
         static const int bk3_f32c[kOrders] = {0, 2, 2, 2, 0, 2, 1, 0, 0};
         static const int bk3_ff[kOrders] = {0, 2, 2, 2, 2, 2, 0, 0, 1};
-        static const int bk1_f32w[kOrders] = {0, 1, 2, 2, 2, 2, 1, 2, 2};
+        static const int bk1_f32w[kOrders] = {0, 9, 4, 9, 4, 4, 4, 4, 4};
         static const int bk1_f32c[kOrders] = {0, 2, 2, 0, 0, 0, 1, 0, 1};
         static const int bk1_ff[kOrders] = {0, 2, 2, 0, 0, 0, 1, 0, 0};
         for (int p = 0; p < kOrders; ++p) {
