@@ -123,23 +123,46 @@ def bk3_workload(p=7, E=65536):
                 E=E)
 
 
+def bk3_even_odd(w, steps, warmup, ms64):
+    """BK3 FP64 with the bases in even-odd form (the GLL-to-Gauss basis and
+    the Gauss collocation derivative are centro-symmetric), beside the
+    drop-in FP64 line of the same workload; error against it."""
+    import torch
+    from omp_bk_amd import ops
+    p = w["p"]
+    P = ops.bk3_basis(p, w["B"], w["D"])
+    ref = ops.bk3(p, w["B"], w["D"], w["G"], w["u"])
+    v = torch.empty_like(w["v"])
+    ms, _ = time_steps(lambda: ops.bk3_prepared(p, P, w["G"], w["u"], out=v), steps, warmup)
+    line = dict(rate(w, ms), speedup_vs_f64=round(ms64 / ms, 3),
+                relerr_vs_f64=float(((v - ref).abs().max() / ref.abs().max()).item()))
+    del v, ref
+    return {"bk3_p%d_f64_eo" % p: line}
+
+
 def bk3_reduced(w, steps, warmup, ms64):
-    """BK3 with u and G stored in FP32 (v FP64), f32w (FP64 arithmetic) and
-    f32c (float-float), beside the FP64 line of the same workload; errors as
-    for the BK5 forms."""
+    """BK3 with u and G stored in FP32 (v FP64), f32w (FP64 arithmetic, with
+    the direct bases and in even-odd form) and f32c (float-float), beside
+    the drop-in FP64 line of the same workload; errors as for the BK5
+    forms."""
     import torch
     from omp_bk_amd import ops
     p, n, q = w["p"], w["p"] + 1, w["p"] + 2
     G32, u32 = w["G"].float(), w["u"].float()
     ref = ops.bk3(p, w["B"], w["D"], w["G"], w["u"])
     ref_q = ops.bk3(p, w["B"], w["D"], G32.double(), u32.double())
+    P = ops.bk3_basis(p, w["B"], w["D"])
     nb = 4 + 4 * 6 * (q / n) ** 3 + 8
     out = {}
-    for form in ("f32w", "f32c"):
+    for form in ("f32w_eo", "f32w", "f32c"):
         v = torch.empty_like(w["v"])
+        if form == "f32w_eo":
+            fn = lambda: ops.bk3_prepared(p, P, G32, u32, out=v, form="f32w")
+        else:
+            fn = lambda: ops.bk3(p, w["B"], w["D"], G32, u32, out=v, form=form)
         # a longer warm-up: the first applies on the freshly converted
         # arrays ran up to 10% slower than the rest
-        ms, _ = time_steps(lambda: ops.bk3(p, w["B"], w["D"], G32, u32, out=v, form=form), steps, max(warmup, 20))
+        ms, _ = time_steps(fn, steps, max(warmup, 20))
         out["bk3_p%d_%s" % (p, form)] = {
             "value": round(w["dofs"] / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
             "speedup_vs_f64": round(ms64 / ms, 3), "bytes_per_dof": round(nb, 3),
@@ -526,6 +549,8 @@ def main():
         w = make()
         ms, _ = time_steps(w["fn"], a.steps, a.warmup)
         lines[name] = rate(w, ms)
+        if name.startswith("bk3_"):
+            lines.update(bk3_even_odd(w, a.steps, a.warmup, ms))
         if name == "bk3_p7_f64":
             lines.update(bk3_reduced(w, a.steps, a.warmup, ms))
         if "out" in w:

@@ -27,6 +27,9 @@ _SIGNATURES = {
     "bkx_bk3_apply_ff": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk3_apply_f32c": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk3_apply_f32w": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk3_basis_prepare": [_c_int, _vp, _vp, _vp],
+    "bkx_bk3_apply_prepared_f64": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk3_apply_prepared_f32w": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk3_apply_kappa_f64": [_c_int, _c_size] + [_vp] * 7,
     "bkx_bk3_adjoint_f64": [_c_int, _c_size] + [_vp] * 10,
     "bkx_bk3_vjp_f64": [_c_int, _c_size] + [_vp] * 9,
@@ -73,6 +76,8 @@ def lib():
         handle = ctypes.CDLL(LIB_PATH)
         handle.bkx_status_string.restype = ctypes.c_char_p
         handle.bkx_status_string.argtypes = [_c_int]
+        handle.bkx_bk3_basis_size.restype = _c_size
+        handle.bkx_bk3_basis_size.argtypes = [_c_int]
         for name, argtypes in _SIGNATURES.items():
             fn = getattr(handle, name)
             fn.argtypes = argtypes
@@ -83,7 +88,7 @@ def lib():
 
 def exported_symbols():
     """Names of the C-ABI functions this binding declares."""
-    return sorted(list(_SIGNATURES) + ["bkx_status_string"])
+    return sorted(list(_SIGNATURES) + ["bkx_status_string", "bkx_bk3_basis_size"])
 
 
 def call(name, *args):

@@ -44,7 +44,8 @@ using bk13t::bk13t_body;
 // in flight through the first four phases).
 constexpr int bk13t_waves(int gpf) { return gpf == 0 ? 1 : (gpf <= 3 ? 4 : (gpf <= 6 ? 3 : 2)); }
 
-template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF, int NT>
+// EO: B and D point at their even-odd forms (EoMat in bk13t_body.hpp).
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF, int NT, bool EO>
 __global__ __launch_bounds__((Bk13tShape<NM, NQ, DIV>::threads))
 __attribute__((amdgpu_waves_per_eu(bk13t_waves(GPF)))) void bk13t_kernel(
     size_t nelmt, const double* __restrict__ B, const double* __restrict__ D, const double* __restrict__ Q,
@@ -58,8 +59,8 @@ __attribute__((amdgpu_waves_per_eu(bk13t_waves(GPF)))) void bk13t_kernel(
     const bool active = slot < S::epb && e < nelmt;
     const size_t ee = active ? e : 0;
     constexpr size_t qper = BK3 ? 6 * S::nq3 : S::nq3;
-    bk13t_body<NM, NQ, BK3, DIV, SB, NB, GPF, BKX_FORM_F64, double, NT>(B, D, Q + ee * qper, u + ee * S::nm3,
-                                                                       v + ee * S::nm3, active, slot, l, s_box);
+    bk13t_body<NM, NQ, BK3, DIV, SB, NB, GPF, BKX_FORM_F64, double, NT, EO>(B, D, Q + ee * qper, u + ee * S::nm3,
+                                                                           v + ee * S::nm3, active, slot, l, s_box);
 }
 
 // Persistent BK3 (PIPE): workgroup w processes the element groups
@@ -131,7 +132,7 @@ int launch_bk3p(size_t nelmt, const double* B, const double* D, const double* Q,
     return launch_status();
 }
 
-template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF = 0, int NT = 0>
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF = 0, int NT = 0, bool EO = false>
 int launch_bk13t(size_t nelmt, const double* B, const double* D, const double* Q, const double* u, double* v,
                  hipStream_t s)
 {
@@ -140,8 +141,32 @@ int launch_bk13t(size_t nelmt, const double* B, const double* D, const double* Q
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk13t_kernel<NM, NQ, BK3, DIV, SB, NB, GPF, NT>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, Q, u, v);
+    hipLaunchKernelGGL((bk13t_kernel<NM, NQ, BK3, DIV, SB, NB, GPF, NT, EO>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, Q, u, v);
     return launch_status();
+}
+
+// BK3 with the bases in even-odd form (P: B's form, then D's): the launch
+// shapes of the default variants 13, 14, 16, 17 and 18; any other variant
+// runs as 16.
+template <int NM>
+int launch_bk3_eo(int variant, size_t nelmt, const double* P, const double* G, const double* u, double* v,
+                  hipStream_t s)
+{
+    constexpr int NQ = NM + 1;
+    const double* B = P;
+    const double* D = P + bk13t::EoMat<NM, NQ, 1>::size;
+    switch (variant) {
+    case 13:
+        return launch_bk13t<NM, NQ, true, 1, false, 2, (NQ < 6 ? NQ : 6), 3, true>(nelmt, B, D, G, u, v, s);
+    case 14:
+        return launch_bk13t<NM, NQ, true, 1, false, 2, NQ, 3, true>(nelmt, B, D, G, u, v, s);
+    case 17:
+        return launch_bk13t<NM, NQ, true, 1, false, 2, 2, 1, true>(nelmt, B, D, G, u, v, s);
+    case 18:
+        return launch_bk13t<NM, NQ, true, 2, false, 2, 3, 1, true>(nelmt, B, D, G, u, v, s);
+    default:
+        return launch_bk13t<NM, NQ, true, 1, false, 2, 3, 1, true>(nelmt, B, D, G, u, v, s);
+    }
 }
 
 template <int NM, bool BK3>
@@ -237,6 +262,12 @@ int bk3t_apply_f64(int variant, int p, size_t nelmt, const double* B, const doub
     return with_order<1, 8>(p, [&](auto P) {
         return launch_bk13t_variant<decltype(P)::value + 1, true>(variant, nelmt, B, D, G, u, v, s);
     });
+}
+
+int bk3t_apply_eo_f64(int variant, int p, size_t nelmt, const double* P, const double* G, const double* u, double* v,
+                      hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto O) { return launch_bk3_eo<decltype(O)::value + 1>(variant, nelmt, P, G, u, v, s); });
 }
 
 }  // namespace bkx

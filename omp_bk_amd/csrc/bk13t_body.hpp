@@ -238,6 +238,151 @@ __device__ __forceinline__ void contract_rows3(P M, const V (&in0)[NIN], V (&out
     }
 }
 
+// ---- even-odd contractions --------------------------------------------------
+// A centro-symmetric (S = 1) or centro-antisymmetric (S = -1) R x C matrix,
+// M[R-1-a][C-1-b] = S M[a][b], as B (GLL to Gauss interpolation, S = 1) and
+// D (Gauss collocation derivative, S = -1) are for symmetric point sets,
+// held in even-odd form (built on the host by bkx_bk3_basis_prepare):
+//   column contraction (out[c] = sum_m M[m][c] in[m]), m < R/2, c < (C+1)/2:
+//     P[m][c] = (M[m][c] + M[R-1-m][c]) / 2,  Q[m][c] = (M[m][c] - M[R-1-m][c]) / 2,
+//     then the middle row M[R/2][c] when R is odd;
+//   row contraction (out[c] = sum_m M[c][m] in[m]), m < C/2, c < (R+1)/2:
+//     P'[m][c] = (M[c][m] + M[c][C-1-m]) / 2,  Q'[m][c] = (M[c][m] - M[c][C-1-m]) / 2,
+//     then the middle column M[c][C/2] when C is odd.
+// With e[m] = in[m] + in[R-1-m] and o[m] = in[m] - in[R-1-m] (or over C),
+// A[c] = sum P e (+ middle), B[c] = sum Q o give out[c] = A + B and the
+// mirrored out[C-1-c] = S (A - B): about 30% fewer operations than the
+// direct contraction (p = 7: 52 against 72 for B, 56 against 81 for D).
+template <int R, int C, int S>
+struct EoMat {
+    static constexpr int hr = R / 2, hc = C / 2, cr = (R + 1) / 2, cc = (C + 1) / 2;
+    static constexpr int cols_p = 0, cols_q = hr * cc, cols_mid = 2 * hr * cc;
+    static constexpr int cols_size = 2 * hr * cc + (R % 2 ? cc : 0);
+    static constexpr int rows_p = cols_size, rows_q = rows_p + hc * cr, rows_mid = rows_p + 2 * hc * cr;
+    static constexpr int size = cols_size + 2 * hc * cr + (C % 2 ? cr : 0);
+    global_ptr<double> p;
+};
+
+template <int NQ, bool SB, int NIN, int NOUT, int R, int C, int S, typename X, typename V>
+__device__ __forceinline__ void contract_cols(EoMat<R, C, S> M, const X (&in)[NIN], V (&out)[NOUT])
+{
+    static_assert(NIN == R && NOUT == C, "the even-odd form contracts a whole matrix");
+    using E = EoMat<R, C, S>;
+    V e[E::hr > 0 ? E::hr : 1], o[E::hr > 0 ? E::hr : 1];
+#pragma unroll
+    for (int m = 0; m < E::hr; ++m) {
+        e[m] = V(in[m]) + V(in[R - 1 - m]);
+        o[m] = V(in[m]) - V(in[R - 1 - m]);
+    }
+#pragma unroll
+    for (int c = 0; c < E::cc; ++c) {
+        const bool mid = (C % 2 == 1) && c == C / 2;
+        Acc<V> a, b;
+        if (!mid || S == 1) {
+#pragma unroll
+            for (int m = 0; m < E::hr; ++m) {
+                a.mac(M.p[E::cols_p + m * E::cc + c], e[m]);
+            }
+            if constexpr (R % 2 == 1) {
+                a.mac(M.p[E::cols_mid + c], V(in[R / 2]));
+            }
+        }
+        if (!mid || S == -1) {
+#pragma unroll
+            for (int m = 0; m < E::hr; ++m) {
+                b.mac(M.p[E::cols_q + m * E::cc + c], o[m]);
+            }
+        }
+        if (mid) {
+            out[c] = S == 1 ? a.get() : b.get();
+        } else {
+            const V A = a.get(), B = b.get();
+            out[c] = A + B;
+            out[C - 1 - c] = S == 1 ? A - B : B - A;
+        }
+    }
+}
+
+template <int NQ, bool SB, int NIN, int NOUT, int R, int C, int S, typename V>
+__device__ __forceinline__ void contract_rows(EoMat<R, C, S> M, const V (&in)[NIN], V (&out)[NOUT])
+{
+    static_assert(NIN == C && NOUT == R, "the even-odd form contracts a whole matrix");
+    using E = EoMat<R, C, S>;
+    V e[E::hc > 0 ? E::hc : 1], o[E::hc > 0 ? E::hc : 1];
+#pragma unroll
+    for (int m = 0; m < E::hc; ++m) {
+        e[m] = in[m] + in[C - 1 - m];
+        o[m] = in[m] - in[C - 1 - m];
+    }
+#pragma unroll
+    for (int c = 0; c < E::cr; ++c) {
+        const bool mid = (R % 2 == 1) && c == R / 2;
+        Acc<V> a, b;
+        if (!mid || S == 1) {
+#pragma unroll
+            for (int m = 0; m < E::hc; ++m) {
+                a.mac(M.p[E::rows_p + m * E::cr + c], e[m]);
+            }
+            if constexpr (C % 2 == 1) {
+                a.mac(M.p[E::rows_mid + c], in[C / 2]);
+            }
+        }
+        if (!mid || S == -1) {
+#pragma unroll
+            for (int m = 0; m < E::hc; ++m) {
+                b.mac(M.p[E::rows_q + m * E::cr + c], o[m]);
+            }
+        }
+        if (mid) {
+            out[c] = S == 1 ? a.get() : b.get();
+        } else {
+            const V A = a.get(), B = b.get();
+            out[c] = A + B;
+            out[R - 1 - c] = S == 1 ? A - B : B - A;
+        }
+    }
+}
+
+template <int NQ, bool SB, int NIN, int NOUT, int R, int C, int S, typename V>
+__device__ __forceinline__ void contract_cols2(EoMat<R, C, S> M, const V (&in0)[NIN], V (&out0)[NOUT],
+                                               const V (&in1)[NIN], V (&out1)[NOUT])
+{
+    contract_cols<NQ, SB>(M, in0, out0);
+    contract_cols<NQ, SB>(M, in1, out1);
+}
+
+template <int NQ, bool SB, int NIN, int NOUT, int R, int C, int S, typename V>
+__device__ __forceinline__ void contract_rows3(EoMat<R, C, S> M, const V (&in0)[NIN], V (&out0)[NOUT],
+                                               const V (&in1)[NIN], V (&out1)[NOUT], const V (&in2)[NIN],
+                                               V (&out2)[NOUT])
+{
+    contract_rows<NQ, SB>(M, in0, out0);
+    contract_rows<NQ, SB>(M, in1, out1);
+    contract_rows<NQ, SB>(M, in2, out2);
+}
+
+// the matrix operand of a contraction with B (nm x nq, centro-symmetric) or
+// D (nq x nq, centro-antisymmetric): rows through the scalar cache, or
+// (EO) the even-odd form at the same pointer
+template <int TAG, bool EO, int NM, int NQ, typename MT>
+__device__ __forceinline__ auto mat_b(const MT* p, int dep)
+{
+    if constexpr (EO) {
+        return EoMat<NM, NQ, 1>{launder<TAG>(p, dep)};
+    } else {
+        return mat<TAG>(p, dep);
+    }
+}
+template <int TAG, bool EO, int NQ, typename MT>
+__device__ __forceinline__ auto mat_d(const MT* p, int dep)
+{
+    if constexpr (EO) {
+        return EoMat<NQ, NQ, -1>{launder<TAG>(p, dep)};
+    } else {
+        return mat<TAG>(p, dep);
+    }
+}
+
 // B (nm x nq) and D (nq x nq) split into float-float in LDS by the whole
 // workgroup (the float-float forms); the caller synchronises afterwards.
 template <int NM, int NQ, bool BK3>
@@ -348,7 +493,8 @@ __device__ __forceinline__ void bk13t_load(const typename Form<FORM>::TS* __rest
     }
 }
 
-template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF, int FORM, typename MT, int NT, bool PIPE>
+template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF, int FORM, typename MT, int NT, bool PIPE,
+          bool EO = false>
 __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT* __restrict__ D,
                                              const typename Form<FORM>::TS* __restrict__ Qe,
                                              double* __restrict__ ve, bool active, int slot, int l, double* smem,
@@ -385,7 +531,7 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
     // ---- P1: lane (j,k), contract i -> p ---------------------------------
     if (in_elem && l < S::nm2) {
         V a[NQ];
-        contract_cols<NQ, SB>(mat<1>(B, dep), xin, a);
+        contract_cols<NQ, SB>(mat_b<1, EO, NM, NQ>(B, dep), xin, a);
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {
             bA[X(p, l_m, l_m2)] = a[p];
@@ -414,7 +560,7 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
             for (int j = 0; j < NM; ++j) {
                 y[j] = bA[X(l_m, j, l_m2)];
             }
-            contract_cols<NQ, SB>(mat<2>(B, dep), y, b);
+            contract_cols<NQ, SB>(mat_b<2, EO, NM, NQ>(B, dep), y, b);
         }
         if constexpr (!BK3 && NB == 2) {
             __syncthreads();
@@ -445,14 +591,14 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
         for (int k = 0; k < NM; ++k) {
             y[k] = bB[X(l_q, l_q2, k)];
         }
-        contract_cols<NQ, SB>(mat<3>(B, dep), y, U);
+        contract_cols<NQ, SB>(mat_b<3, EO, NM, NQ>(B, dep), y, U);
         if constexpr (!BK3) {
 #pragma unroll
             for (int r = 0; r < NQ; ++r) {
                 W[r] = mul(U[r], bJ[X(l_q, l_q2, r)]);
             }
         } else {
-            contract_cols<NQ, SB>(mat<4>(D, dep), U, W);   // qt, held in W
+            contract_cols<NQ, SB>(mat_d<4, EO, NQ>(D, dep), U, W);   // qt, held in W
 #pragma unroll
             for (int r = 0; r < NQ; ++r) {
                 bA[X(l_q, l_q2, r)] = U[r];
@@ -485,7 +631,7 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
                         y[q] = bA[X(l_q, q, l_q2)];      // lane (p, r), col q
                         z[q] = bA[X(q, l_q, l_q2)];      // lane (q, r), col p
                     }
-                    contract_cols2<NQ, SB>(mat<5>(D, dep), y, qs, z, qr);
+                    contract_cols2<NQ, SB>(mat_d<5, EO, NQ>(D, dep), y, qs, z, qr);
                 } else {
                     // one input at a time (fewer registers live; D's rows
                     // are read twice from the scalar cache)
@@ -493,12 +639,12 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
                     for (int q = 0; q < NQ; ++q) {
                         y[q] = bA[X(l_q, q, l_q2)];
                     }
-                    contract_cols<NQ, SB>(mat<5>(D, dep), y, qs);
+                    contract_cols<NQ, SB>(mat_d<5, EO, NQ>(D, dep), y, qs);
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) {
                         z[q] = bA[X(q, l_q, l_q2)];
                     }
-                    contract_cols<NQ, SB>(mat<6>(D, dep), z, qr);
+                    contract_cols<NQ, SB>(mat_d<6, EO, NQ>(D, dep), z, qr);
                 }
             }
             if constexpr (NB == 2) {
@@ -604,19 +750,19 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
                         y[q] = bS[X(l_q, q, l_q2)];      // rs, lane (p, r), col q
                         z[q] = bC[X(l_q, l_q2, q)];      // rt, lane (p, q), col r
                     }
-                    contract_rows3<NQ, SB>(mat<7>(D, dep), rr, Wr, y, Ws, z, Wt);
+                    contract_rows3<NQ, SB>(mat_d<7, EO, NQ>(D, dep), rr, Wr, y, Ws, z, Wt);
                 } else {
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) {
                         z[q] = bC[X(l_q, l_q2, q)];
                     }
-                    contract_rows<NQ, SB>(mat<7>(D, dep), z, Wt);
+                    contract_rows<NQ, SB>(mat_d<7, EO, NQ>(D, dep), z, Wt);
 #pragma unroll
                     for (int q = 0; q < NQ; ++q) {
                         y[q] = bS[X(l_q, q, l_q2)];
                     }
-                    contract_rows<NQ, SB>(mat<8>(D, dep), y, Ws);
-                    contract_rows<NQ, SB>(mat<9>(D, dep), rr, Wr);
+                    contract_rows<NQ, SB>(mat_d<8, EO, NQ>(D, dep), y, Ws);
+                    contract_rows<NQ, SB>(mat_d<9, EO, NQ>(D, dep), rr, Wr);
                 }
             }
             if constexpr (NB == 2) {
@@ -651,7 +797,7 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
     // ---- P7: lane (p,q), contract r -> k ---------------------------------
     if (in_elem) {
         V Y[NM];
-        contract_rows<NQ, SB>(mat<10>(B, dep), W, Y);
+        contract_rows<NQ, SB>(mat_b<10, EO, NM, NQ>(B, dep), W, Y);
 #pragma unroll
         for (int k = 0; k < NM; ++k) {
             bA[X(l_q, l_q2, k)] = Y[k];
@@ -666,7 +812,7 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
         for (int q = 0; q < NQ; ++q) {
             y[q] = bA[X(l_m, q, l_m2)];
         }
-        contract_rows<NQ, SB>(mat<11>(B, dep), y, Z);
+        contract_rows<NQ, SB>(mat_b<11, EO, NM, NQ>(B, dep), y, Z);
 #pragma unroll
         for (int j = 0; j < NM; ++j) {
             bB[X(l_m, j, l_m2)] = Z[j];
@@ -681,7 +827,7 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
         for (int p = 0; p < NQ; ++p) {
             z[p] = bB[X(p, l_m, l_m2)];
         }
-        contract_rows<NQ, SB>(mat<12>(B, dep), z, o);
+        contract_rows<NQ, SB>(mat_b<12, EO, NM, NQ>(B, dep), z, o);
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
             store_v<(NT & 1) != 0>(ve + i * S::nm2 + l, o[i]);
@@ -690,7 +836,7 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
 }
 
 template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB = 3, int GPF = 0, int FORM = BKX_FORM_F64,
-          typename MT = double, int NT = 0>
+          typename MT = double, int NT = 0, bool EO = false>
 __device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* __restrict__ D,
                                            const typename Form<FORM>::TS* __restrict__ Qe,
                                            const typename Form<FORM>::TS* __restrict__ ue,
@@ -700,8 +846,8 @@ __device__ __forceinline__ void bk13t_body(const MT* __restrict__ B, const MT* _
     typename Form<FORM>::L xin[NM];
     typename Form<FORM>::L qin[bk13t_nqv(BK3, GPF, NQ)];
     bk13t_load<NM, NQ, BK3, DIV, GPF, FORM, NT, false>(Qe, ue, slot, l, xin, qin);
-    bk13t_phases<NM, NQ, BK3, DIV, SB, NB, GPF, FORM, MT, NT, false>(B, D, Qe, ve, active, slot, l, smem, dep, xin, qin,
-                                                                     nullptr, nullptr);
+    bk13t_phases<NM, NQ, BK3, DIV, SB, NB, GPF, FORM, MT, NT, false, EO>(B, D, Qe, ve, active, slot, l, smem, dep, xin,
+                                                                         qin, nullptr, nullptr);
 }
 
 }  // namespace bk13t

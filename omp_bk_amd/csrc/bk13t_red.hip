@@ -23,7 +23,8 @@ namespace {
 
 using bk13t::Bk13tShape;
 
-template <int NM, int NQ, bool BK3, int DIV, int NB, int GPF, int FORM, int NT>
+// EO (FP64-arithmetic forms): B and D point at their even-odd forms.
+template <int NM, int NQ, bool BK3, int DIV, int NB, int GPF, int FORM, int NT, bool EO>
 __global__ __launch_bounds__((Bk13tShape<NM, NQ, DIV>::threads))
 __attribute__((amdgpu_waves_per_eu(GPF > 0 ? 4 : 1))) void bk13t_red_kernel(
     size_t nelmt, const double* __restrict__ B, const double* __restrict__ D,
@@ -48,12 +49,12 @@ __attribute__((amdgpu_waves_per_eu(GPF > 0 ? 4 : 1))) void bk13t_red_kernel(
         bk13t::bk13t_body<NM, NQ, BK3, DIV, true, NB, GPF, FORM, ff>(
             s_mat, s_mat + NM * NQ, Q + ee * qper, u + ee * S::nm3, v + ee * S::nm3, active, slot, l, s_box);
     } else {
-        bk13t::bk13t_body<NM, NQ, BK3, DIV, false, NB, GPF, FORM, double, NT>(
+        bk13t::bk13t_body<NM, NQ, BK3, DIV, false, NB, GPF, FORM, double, NT, EO>(
             B, D, Q + ee * qper, u + ee * S::nm3, v + ee * S::nm3, active, slot, l, s_box);
     }
 }
 
-template <int NM, bool BK3, int DIV, int NB, int GPF, int FORM, int NT = 0>
+template <int NM, bool BK3, int DIV, int NB, int GPF, int FORM, int NT = 0, bool EO = false>
 int launch_red(size_t nelmt, const double* B, const double* D, const void* Q, const void* u, double* v,
                hipStream_t s)
 {
@@ -63,7 +64,7 @@ int launch_red(size_t nelmt, const double* B, const double* D, const void* Q, co
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk13t_red_kernel<NM, NM + 1, BK3, DIV, NB, GPF, FORM, NT>), dim3(grid), dim3(S::threads), 0, s,
+    hipLaunchKernelGGL((bk13t_red_kernel<NM, NM + 1, BK3, DIV, NB, GPF, FORM, NT, EO>), dim3(grid), dim3(S::threads), 0, s,
                        nelmt, B, D, static_cast<const TS*>(Q), static_cast<const TS*>(u), v);
     return launch_status();
 }
@@ -157,6 +158,30 @@ int bk13_red_variants(int form)
     default:
         return -1;
     }
+}
+
+// BK3 f32w with the bases in even-odd form (P: B's form, then D's): the
+// launch shapes of the default variants 4, 6, 8 and 9; any other variant
+// runs as 8.
+int bk3_apply_f32w_eo(int variant, int p, size_t nelmt, const double* P, const float* G, const float* u, double* v,
+                      hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto O) {
+        constexpr int NM = decltype(O)::value + 1;
+        constexpr int F = BKX_FORM_F32W;
+        const double* B = P;
+        const double* D = P + bk13t::EoMat<NM, NM + 1, 1>::size;
+        switch (variant) {
+        case 4:
+            return launch_red<NM, true, 1, 2, 0, F, 1, true>(nelmt, B, D, G, u, v, s);
+        case 6:
+            return launch_red<NM, true, 1, 2, 2, F, 1, true>(nelmt, B, D, G, u, v, s);
+        case 9:
+            return launch_red<NM, true, 2, 2, 3, F, 1, true>(nelmt, B, D, G, u, v, s);
+        default:
+            return launch_red<NM, true, 1, 2, 3, F, 1, true>(nelmt, B, D, G, u, v, s);
+        }
+    });
 }
 
 int bk1_apply_red(int form, int variant, int p, size_t nelmt, const double* B, const void* JxW, const void* u,

@@ -4,6 +4,9 @@
 
 #include <atomic>
 
+#include <cmath>
+
+#include "bk13t_body.hpp"
 #include "bkx.h"
 #include "bkx_launch.hpp"
 
@@ -46,6 +49,10 @@ int bk1_apply_red(int form, int variant, int p, size_t nelmt, const double* B, c
                   double* v, hipStream_t s);
 int bk3_apply_red(int form, int variant, int p, size_t nelmt, const double* B, const double* D, const void* G,
                   const void* u, double* v, hipStream_t s);
+int bk3t_apply_eo_f64(int variant, int p, size_t nelmt, const double* P, const double* G, const double* u, double* v,
+                      hipStream_t s);
+int bk3_apply_f32w_eo(int variant, int p, size_t nelmt, const double* P, const float* G, const float* u, double* v,
+                      hipStream_t s);
 }  // namespace bkx
 
 namespace {
@@ -109,6 +116,48 @@ bool tune_index_ok(int op, int form, int p)
 }
 
 int variant_of(int op, int form, int p) { return g_variant[op][form][p].load(std::memory_order_relaxed); }
+
+// The even-odd form (bkx::bk13t::EoMat) of the row-major R x C matrix M,
+// after checking M[R-1-a][C-1-b] = S M[a][b] to 1e-14 of max |M|.
+template <int R, int C, int S>
+bool eo_form(const double* M, double* out)
+{
+    using E = bkx::bk13t::EoMat<R, C, S>;
+    double big = 0.0;
+    for (int x = 0; x < R * C; ++x) {
+        big = std::fmax(big, std::fabs(M[x]));
+    }
+    for (int a = 0; a < R; ++a) {
+        for (int b = 0; b < C; ++b) {
+            if (!(std::fabs(M[(R - 1 - a) * C + (C - 1 - b)] - S * M[a * C + b]) <= 1e-14 * big)) {
+                return false;
+            }
+        }
+    }
+    for (int m = 0; m < E::hr; ++m) {
+        for (int c = 0; c < E::cc; ++c) {
+            out[E::cols_p + m * E::cc + c] = 0.5 * (M[m * C + c] + M[(R - 1 - m) * C + c]);
+            out[E::cols_q + m * E::cc + c] = 0.5 * (M[m * C + c] - M[(R - 1 - m) * C + c]);
+        }
+    }
+    if (R % 2 == 1) {
+        for (int c = 0; c < E::cc; ++c) {
+            out[E::cols_mid + c] = M[(R / 2) * C + c];
+        }
+    }
+    for (int m = 0; m < E::hc; ++m) {
+        for (int c = 0; c < E::cr; ++c) {
+            out[E::rows_p + m * E::cr + c] = 0.5 * (M[c * C + m] + M[c * C + (C - 1 - m)]);
+            out[E::rows_q + m * E::cr + c] = 0.5 * (M[c * C + m] - M[c * C + (C - 1 - m)]);
+        }
+    }
+    if (C % 2 == 1) {
+        for (int c = 0; c < E::cr; ++c) {
+            out[E::rows_mid + c] = M[c * C + C / 2];
+        }
+    }
+    return true;
+}
 
 }  // namespace
 
@@ -285,6 +334,64 @@ int bkx_bk3_apply_f32w(int p, size_t nelmt, const double* basis, const double* d
                        const float* in, double* out, bkx_stream_t stream)
 {
     return bk3_red(BKX_FORM_F32W, p, nelmt, basis, dbasis, G, in, out, stream);
+}
+
+size_t bkx_bk3_basis_size(int p)
+{
+    if (!order_ok(3, p)) {
+        return 0;
+    }
+    return size_t(bkx::with_order<1, 8>(p, [](auto O) {
+        constexpr int NM = decltype(O)::value + 1;
+        return bkx::bk13t::EoMat<NM, NM + 1, 1>::size + bkx::bk13t::EoMat<NM + 1, NM + 1, -1>::size;
+    }));
+}
+
+int bkx_bk3_basis_prepare(int p, const double* basis, const double* dbasis, double* prepared)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (!basis || !dbasis || !prepared) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::with_order<1, 8>(p, [&](auto O) {
+        constexpr int NM = decltype(O)::value + 1;
+        constexpr int NQ = NM + 1;
+        const bool ok = eo_form<NM, NQ, 1>(basis, prepared) &&
+                        eo_form<NQ, NQ, -1>(dbasis, prepared + bkx::bk13t::EoMat<NM, NQ, 1>::size);
+        return ok ? int(BKX_SUCCESS) : int(BKX_ERROR_ARG);
+    });
+}
+
+int bkx_bk3_apply_prepared_f64(int p, size_t nelmt, const double* prepared, const double* G, const double* in,
+                               double* out, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!prepared || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3t_apply_eo_f64(variant_of(3, BKX_FORM_F64, p), p, nelmt, prepared, G, in, out, as_stream(stream));
+}
+
+int bkx_bk3_apply_prepared_f32w(int p, size_t nelmt, const double* prepared, const float* G, const float* in,
+                                double* out, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!prepared || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_apply_f32w_eo(variant_of(3, BKX_FORM_F32W, p), p, nelmt, prepared, G, in, out, as_stream(stream));
 }
 
 int bkx_bk3_apply_f32c(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,

@@ -11,6 +11,7 @@ current torch stream through the C ABI of include/bkx.h.  Like the reference
 the calls return once the work is enqueued on the stream; torch's stream
 semantics order later work after it.
 """
+import numpy as np
 import torch
 
 from . import _lib
@@ -108,6 +109,53 @@ def bk3(p, basis, dbasis, G, u, out=None, form="f64"):
     with torch.cuda.device(u.device):
         _lib.call(entry, p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(),
                   _stream(u))
+    return out
+
+
+def bk3_basis(p, basis, dbasis, device=None):
+    """The even-odd form of centro-symmetric BK3 bases (B(i, p) =
+    B(n-1-i, q-1-p), D(m, p) = -D(q-1-m, q-1-p): the GLL-to-Gauss basis and
+    the Gauss collocation derivative), built on the host by
+    bkx_bk3_basis_prepare, as an FP64 tensor on ``device`` (default: that of
+    ``basis`` when it is a tensor).  Raises ValueError when the bases are not
+    centro-symmetric."""
+    _order_ok(3, p)
+    if device is None and isinstance(basis, torch.Tensor):
+        device = basis.device
+    b = np.ascontiguousarray(torch.as_tensor(basis).detach().cpu().numpy(), dtype=np.float64).reshape(-1)
+    d = np.ascontiguousarray(torch.as_tensor(dbasis).detach().cpu().numpy(), dtype=np.float64).reshape(-1)
+    nm, nq = p + 1, p + 2
+    if b.size != nm * nq or d.size != nq * nq:
+        raise ValueError("bases of order %d are %d x %d and %d x %d" % (p, nm, nq, nq, nq))
+    out = np.empty(_lib.lib().bkx_bk3_basis_size(p))
+    st = _lib.lib().bkx_bk3_basis_prepare(p, b.ctypes.data, d.ctypes.data, out.ctypes.data)
+    if st == 5:
+        raise ValueError("the bases are not centro-symmetric: use bk3 with basis and dbasis")
+    if st != 0:
+        raise _lib.BkxError("bkx_bk3_basis_prepare", st, _lib.lib().bkx_status_string(st).decode())
+    return torch.as_tensor(out, device=device)
+
+
+def bk3_prepared(p, prepared, G, u, out=None, form="f64"):
+    """BK3 as bk3, with the bases in the even-odd form of bk3_basis; form
+    "f64" or "f32w".  The result agrees with bk3's to rounding."""
+    _order_ok(3, p)
+    entry = {"f64": "bkx_bk3_apply_prepared_f64", "f32w": "bkx_bk3_apply_prepared_f32w"}.get(form)
+    if entry is None:
+        raise ValueError("form must be f64 or f32w")
+    sdt = torch.float64 if form == "f64" else torch.float32
+    nm, nq = p + 1, p + 2
+    if u.numel() % nm ** 3:
+        raise ValueError("u must hold whole elements of %d values" % nm ** 3)
+    ne = u.numel() // nm ** 3
+    _check("prepared", prepared, _lib.lib().bkx_bk3_basis_size(p))
+    _check("G", G, ne * 6 * nq ** 3, sdt)
+    _check("u", u, ne * nm ** 3, sdt)
+    if out is None:
+        out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
+    _check("out", out, ne * nm ** 3)
+    with torch.cuda.device(u.device):
+        _lib.call(entry, p, ne, prepared.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
     return out
 
 
