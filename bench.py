@@ -320,6 +320,15 @@ def adjoint_config(steps, warmup, p=6, E=16384):
             line["relerr_vs_f64_same_inputs"] = {"L": abs(L.item() - ref[0].item()) / ref[0].item(),
                                                  "dLdu": rel(du, ref[1]), "dLdkappa": rel(dk, ref[2])}
         out["bk3_adjoint_p6_" + form] = line
+        # the same with the bases in even-odd form
+        P = ops.bk3_basis(p, B, D)
+        ms_eo, _ = time_steps(lambda: ops.bk3_adjoint_prepared(p, P, Gs, ks, us, bs, form=form), steps, warmup)
+        r_eo, r_d = ops.bk3_adjoint_prepared(p, P, Gs, ks, us, bs, form=form), res["r"]
+        rel = lambda x, r: float(((x - r).abs().max() / r.abs().max()).item())
+        out["bk3_adjoint_p6_%s_eo" % form] = {
+            "value": round(dofs / (ms_eo * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms_eo, 5),
+            "bytes_per_dof": round(nbytes / dofs, 2), "achieved_GBs": round(nbytes / (ms_eo * 1e-3) * 1e-9, 1),
+            "relerr_vs_direct": {"dLdu": rel(r_eo[1], r_d[1]), "dLdkappa": rel(r_eo[2], r_d[2])}}
         del Gs, ks, us, bs
     return out
 

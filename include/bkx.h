@@ -175,6 +175,13 @@ int bkx_bk3_apply_prepared_f64(int p, size_t nelmt, const double* prepared, cons
                                double* out, bkx_stream_t stream);
 int bkx_bk3_apply_prepared_f32w(int p, size_t nelmt, const double* prepared, const float* G, const float* in,
                                 double* out, bkx_stream_t stream);
+/* bkx_bk3_adjoint_{f64,f32w} with the bases in the prepared even-odd form. */
+int bkx_bk3_adjoint_prepared_f64(int p, size_t nelmt, const double* prepared, const double* G, const double* kappa,
+                                 const double* u, const double* b, double* dldu, double* dldkappa, double* loss,
+                                 bkx_stream_t stream);
+int bkx_bk3_adjoint_prepared_f32w(int p, size_t nelmt, const double* prepared, const float* G, const float* kappa,
+                                  const float* u, const float* b, double* dldu, double* dldkappa, double* loss,
+                                  bkx_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
 /* BK3 with a coefficient and its adjoint (no reference counterpart; the    */

@@ -30,6 +30,8 @@ _SIGNATURES = {
     "bkx_bk3_basis_prepare": [_c_int, _vp, _vp, _vp],
     "bkx_bk3_apply_prepared_f64": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk3_apply_prepared_f32w": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk3_adjoint_prepared_f64": [_c_int, _c_size] + [_vp] * 9,
+    "bkx_bk3_adjoint_prepared_f32w": [_c_int, _c_size] + [_vp] * 9,
     "bkx_bk3_apply_kappa_f64": [_c_int, _c_size] + [_vp] * 7,
     "bkx_bk3_adjoint_f64": [_c_int, _c_size] + [_vp] * 10,
     "bkx_bk3_vjp_f64": [_c_int, _c_size] + [_vp] * 9,

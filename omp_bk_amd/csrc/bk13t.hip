@@ -44,7 +44,7 @@ using bk13t::bk13t_body;
 // in flight through the first four phases).
 constexpr int bk13t_waves(int gpf) { return gpf == 0 ? 1 : (gpf <= 3 ? 4 : (gpf <= 6 ? 3 : 2)); }
 
-// EO: B and D point at their even-odd forms (EoMat in bk13t_body.hpp).
+// EO: B and D point at their even-odd forms (EoMat in bkx_eo.hpp).
 template <int NM, int NQ, bool BK3, int DIV, bool SB, int NB, int GPF, int NT, bool EO>
 __global__ __launch_bounds__((Bk13tShape<NM, NQ, DIV>::threads))
 __attribute__((amdgpu_waves_per_eu(bk13t_waves(GPF)))) void bk13t_kernel(
@@ -154,7 +154,7 @@ int launch_bk3_eo(int variant, size_t nelmt, const double* P, const double* G, c
 {
     constexpr int NQ = NM + 1;
     const double* B = P;
-    const double* D = P + bk13t::EoMat<NM, NQ, 1>::size;
+    const double* D = P + EoMat<NM, NQ, 1>::size;
     switch (variant) {
     case 13:
         return launch_bk13t<NM, NQ, true, 1, false, 2, (NQ < 6 ? NQ : 6), 3, true>(nelmt, B, D, G, u, v, s);

@@ -6,6 +6,7 @@
 
 #include "bkx.h"
 #include "bkx_device.hpp"
+#include "bkx_eo.hpp"
 #include "bkx_ff.hpp"
 
 namespace bkx {
@@ -238,31 +239,7 @@ __device__ __forceinline__ void contract_rows3(P M, const V (&in0)[NIN], V (&out
     }
 }
 
-// ---- even-odd contractions --------------------------------------------------
-// A centro-symmetric (S = 1) or centro-antisymmetric (S = -1) R x C matrix,
-// M[R-1-a][C-1-b] = S M[a][b], as B (GLL to Gauss interpolation, S = 1) and
-// D (Gauss collocation derivative, S = -1) are for symmetric point sets,
-// held in even-odd form (built on the host by bkx_bk3_basis_prepare):
-//   column contraction (out[c] = sum_m M[m][c] in[m]), m < R/2, c < (C+1)/2:
-//     P[m][c] = (M[m][c] + M[R-1-m][c]) / 2,  Q[m][c] = (M[m][c] - M[R-1-m][c]) / 2,
-//     then the middle row M[R/2][c] when R is odd;
-//   row contraction (out[c] = sum_m M[c][m] in[m]), m < C/2, c < (R+1)/2:
-//     P'[m][c] = (M[c][m] + M[c][C-1-m]) / 2,  Q'[m][c] = (M[c][m] - M[c][C-1-m]) / 2,
-//     then the middle column M[c][C/2] when C is odd.
-// With e[m] = in[m] + in[R-1-m] and o[m] = in[m] - in[R-1-m] (or over C),
-// A[c] = sum P e (+ middle), B[c] = sum Q o give out[c] = A + B and the
-// mirrored out[C-1-c] = S (A - B): about 30% fewer operations than the
-// direct contraction (p = 7: 52 against 72 for B, 56 against 81 for D).
-template <int R, int C, int S>
-struct EoMat {
-    static constexpr int hr = R / 2, hc = C / 2, cr = (R + 1) / 2, cc = (C + 1) / 2;
-    static constexpr int cols_p = 0, cols_q = hr * cc, cols_mid = 2 * hr * cc;
-    static constexpr int cols_size = 2 * hr * cc + (R % 2 ? cc : 0);
-    static constexpr int rows_p = cols_size, rows_q = rows_p + hc * cr, rows_mid = rows_p + 2 * hc * cr;
-    static constexpr int size = cols_size + 2 * hc * cr + (C % 2 ? cr : 0);
-    global_ptr<double> p;
-};
-
+// ---- even-odd contractions (EoMat, bkx_eo.hpp) -----------------------------
 template <int NQ, bool SB, int NIN, int NOUT, int R, int C, int S, typename X, typename V>
 __device__ __forceinline__ void contract_cols(EoMat<R, C, S> M, const X (&in)[NIN], V (&out)[NOUT])
 {

@@ -170,7 +170,7 @@ int bk3_apply_f32w_eo(int variant, int p, size_t nelmt, const double* P, const f
         constexpr int NM = decltype(O)::value + 1;
         constexpr int F = BKX_FORM_F32W;
         const double* B = P;
-        const double* D = P + bk13t::EoMat<NM, NM + 1, 1>::size;
+        const double* D = P + EoMat<NM, NM + 1, 1>::size;
         switch (variant) {
         case 4:
             return launch_red<NM, true, 1, 2, 0, F, 1, true>(nelmt, B, D, G, u, v, s);

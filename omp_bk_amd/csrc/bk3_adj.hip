@@ -47,7 +47,8 @@ struct AdjShape {
     static constexpr int threads = ((epb * nq2 + kWave - 1) / kWave) * kWave;
 };
 
-template <int NM, int NQ, int MODE, typename TS>
+// EO: Bm and Dm point at the even-odd forms of B and D (bkx_eo.hpp).
+template <int NM, int NQ, int MODE, typename TS, bool EO>
 __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
     size_t nelmt, const double* __restrict__ Bm, const double* __restrict__ Dm, const TS* __restrict__ G,
     const TS* __restrict__ kappa, const TS* __restrict__ u, const TS* __restrict__ bw,
@@ -90,7 +91,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
     double x[NM];
     modal_load(u, x);
     double qr[NQ];
-    el.template grad<0>(Bm, Dm, x, qr);
+    el.template grad<0, EO>(Bm, Dm, x, qr);
 
     // F = G P u (kept), kappa F -> rr (registers), rs -> B, rt -> C
     const TS* ge = G + ee * 6 * E::nq3 + l;      // G(e, f, p, q, r) at f = p = 0
@@ -129,7 +130,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
         modal_load(bw, r);
     } else {
         double o[NM];
-        el.template div<10>(Bm, Dm, rr, o);
+        el.template div<10, EO>(Bm, Dm, rr, o);
         if constexpr (MODE == kApply) {
             if (active && modal) {
 #pragma unroll
@@ -166,7 +167,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
 
     // gradient of r, dL/dkappa = P r . F, kappa G P r
     double qrr[NQ];
-    el.template grad<20>(Bm, Dm, r, qrr);
+    el.template grad<20, EO>(Bm, Dm, r, qrr);
     if (in_elem) {
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {
@@ -195,7 +196,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
     }
     __syncthreads();
     double o2[NM];
-    el.template div<30>(Bm, Dm, rr, o2);
+    el.template div<30, EO>(Bm, Dm, rr, o2);
     if (active && modal) {
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
@@ -204,7 +205,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
     }
 }
 
-template <int NM, int MODE, typename TS>
+template <int NM, int MODE, typename TS, bool EO = false>
 int launch_adj(size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
                const TS* bw, double* out_u, double* out_k, double* loss, hipStream_t s)
 {
@@ -213,8 +214,8 @@ int launch_adj(size_t nelmt, const double* B, const double* D, const TS* G, cons
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk3_adj_kernel<NM, NM + 1, MODE, TS>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, G, kappa,
-                       u, bw, out_u, out_k, loss);
+    hipLaunchKernelGGL((bk3_adj_kernel<NM, NM + 1, MODE, TS, EO>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, G,
+                       kappa, u, bw, out_u, out_k, loss);
     return launch_status();
 }
 
@@ -263,6 +264,18 @@ int bk3_adjoint(int p, size_t nelmt, const double* B, const double* D, const TS*
     });
 }
 
+// The loss mode with the bases in even-odd form (P: B's form, then D's).
+template <typename TS>
+int bk3_adjoint_eo(int p, size_t nelmt, const double* P, const TS* G, const TS* kappa, const TS* u, const TS* b,
+                   double* dldu, double* dldk, double* loss, hipStream_t s)
+{
+    return with_order<1, 8>(p, [&](auto O) {
+        constexpr int NM = decltype(O)::value + 1;
+        return launch_adj<NM, kLoss, TS, true>(nelmt, P, P + EoMat<NM, NM + 1, 1>::size, G, kappa, u, b, dldu, dldk,
+                                               loss, s);
+    });
+}
+
 template <typename TS>
 int bk3_vjp(int p, size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
             const TS* w, double* gu, double* gk, hipStream_t s)
@@ -306,6 +319,18 @@ int bk3_vjp_f32w(int p, size_t nelmt, const double* B, const double* D, const fl
                  const float* u, const float* w, double* gu, double* gk, hipStream_t s)
 {
     return bk3_vjp(p, nelmt, B, D, G, kappa, u, w, gu, gk, s);
+}
+
+int bk3_adjoint_eo_f64(int p, size_t nelmt, const double* P, const double* G, const double* kappa, const double* u,
+                       const double* b, double* dldu, double* dldk, double* loss, hipStream_t s)
+{
+    return bk3_adjoint_eo(p, nelmt, P, G, kappa, u, b, dldu, dldk, loss, s);
+}
+
+int bk3_adjoint_eo_f32w(int p, size_t nelmt, const double* P, const float* G, const float* kappa, const float* u,
+                        const float* b, double* dldu, double* dldk, double* loss, hipStream_t s)
+{
+    return bk3_adjoint_eo(p, nelmt, P, G, kappa, u, b, dldu, dldk, loss, s);
 }
 
 int sum_f64(const double* x, size_t n, double* out, hipStream_t s)

@@ -17,6 +17,7 @@
 #pragma once
 
 #include "bkx_device.hpp"
+#include "bkx_eo.hpp"
 
 namespace bkx {
 namespace bk3p {
@@ -100,6 +101,124 @@ __device__ __forceinline__ void rows3(P M, const double (&in0)[NIN], double (&ou
     }
 }
 
+// The same contractions with the matrix in even-odd form (bkx_eo.hpp).
+template <int LD, int NIN, int NOUT, int R, int C, int S>
+__device__ __forceinline__ void cols(EoMat<R, C, S> M, const double (&in)[NIN], double (&out)[NOUT])
+{
+    static_assert(NIN == R && NOUT == C, "the even-odd form contracts a whole matrix");
+    using E = EoMat<R, C, S>;
+    double e[E::hr > 0 ? E::hr : 1], o[E::hr > 0 ? E::hr : 1];
+#pragma unroll
+    for (int m = 0; m < E::hr; ++m) {
+        e[m] = in[m] + in[R - 1 - m];
+        o[m] = in[m] - in[R - 1 - m];
+    }
+#pragma unroll
+    for (int c = 0; c < E::cc; ++c) {
+        const bool mid = (C % 2 == 1) && c == C / 2;
+        double a = 0.0, b = 0.0;
+        if (!mid || S == 1) {
+#pragma unroll
+            for (int m = 0; m < E::hr; ++m) {
+                a += M.p[E::cols_p + m * E::cc + c] * e[m];
+            }
+            if constexpr (R % 2 == 1) {
+                a += M.p[E::cols_mid + c] * in[R / 2];
+            }
+        }
+        if (!mid || S == -1) {
+#pragma unroll
+            for (int m = 0; m < E::hr; ++m) {
+                b += M.p[E::cols_q + m * E::cc + c] * o[m];
+            }
+        }
+        if (mid) {
+            out[c] = S == 1 ? a : b;
+        } else {
+            out[c] = a + b;
+            out[C - 1 - c] = S == 1 ? a - b : b - a;
+        }
+    }
+}
+
+template <int LD, int NIN, int NOUT, int R, int C, int S>
+__device__ __forceinline__ void cols2(EoMat<R, C, S> M, const double (&in0)[NIN], double (&out0)[NOUT],
+                                      const double (&in1)[NIN], double (&out1)[NOUT])
+{
+    cols<LD>(M, in0, out0);
+    cols<LD>(M, in1, out1);
+}
+
+template <int LD, int NIN, int NOUT, int R, int C, int S>
+__device__ __forceinline__ void rows(EoMat<R, C, S> M, const double (&in)[NIN], double (&out)[NOUT])
+{
+    static_assert(NIN == C && NOUT == R, "the even-odd form contracts a whole matrix");
+    using E = EoMat<R, C, S>;
+    double e[E::hc > 0 ? E::hc : 1], o[E::hc > 0 ? E::hc : 1];
+#pragma unroll
+    for (int m = 0; m < E::hc; ++m) {
+        e[m] = in[m] + in[C - 1 - m];
+        o[m] = in[m] - in[C - 1 - m];
+    }
+#pragma unroll
+    for (int c = 0; c < E::cr; ++c) {
+        const bool mid = (R % 2 == 1) && c == R / 2;
+        double a = 0.0, b = 0.0;
+        if (!mid || S == 1) {
+#pragma unroll
+            for (int m = 0; m < E::hc; ++m) {
+                a += M.p[E::rows_p + m * E::cr + c] * e[m];
+            }
+            if constexpr (C % 2 == 1) {
+                a += M.p[E::rows_mid + c] * in[C / 2];
+            }
+        }
+        if (!mid || S == -1) {
+#pragma unroll
+            for (int m = 0; m < E::hc; ++m) {
+                b += M.p[E::rows_q + m * E::cr + c] * o[m];
+            }
+        }
+        if (mid) {
+            out[c] = S == 1 ? a : b;
+        } else {
+            out[c] = a + b;
+            out[R - 1 - c] = S == 1 ? a - b : b - a;
+        }
+    }
+}
+
+template <int LD, int NIN, int NOUT, int R, int C, int S>
+__device__ __forceinline__ void rows3(EoMat<R, C, S> M, const double (&in0)[NIN], double (&out0)[NOUT],
+                                      const double (&in1)[NIN], double (&out1)[NOUT],
+                                      const double (&in2)[NIN], double (&out2)[NOUT])
+{
+    rows<LD>(M, in0, out0);
+    rows<LD>(M, in1, out1);
+    rows<LD>(M, in2, out2);
+}
+
+// the operand of a contraction with B (EO: the even-odd form of the
+// nm x nq B at that pointer) or D (of the nq x nq D)
+template <int TAG, bool EO, int NM, int NQ>
+__device__ __forceinline__ auto op_b(const double* p)
+{
+    if constexpr (EO) {
+        return EoMat<NM, NQ, 1>{launder<TAG>(p)};
+    } else {
+        return launder<TAG>(p);
+    }
+}
+template <int TAG, bool EO, int NQ>
+__device__ __forceinline__ auto op_d(const double* p)
+{
+    if constexpr (EO) {
+        return EoMat<NQ, NQ, -1>{launder<TAG>(p)};
+    } else {
+        return launder<TAG>(p);
+    }
+}
+
 template <int NM, int NQ>
 struct Elem {
     static constexpr int nm2 = NM * NM;
@@ -134,12 +253,12 @@ struct Elem {
 
     // modal x (lane (j,k), l < nm^2) -> qr (lane (q,r)), qs -> B, qt -> C.
     // Ends with the workgroup synchronised.
-    template <int TAG>
+    template <int TAG, bool EO = false>
     __device__ void grad(const double* Bm, const double* Dm, const double (&x)[NM], double (&qr)[NQ])
     {
         if (in_elem && l < nm2) {
             double a[NQ];
-            cols<NQ>(launder<TAG + 1>(Bm), x, a);
+            cols<NQ>(op_b<TAG + 1, EO, NM, NQ>(Bm), x, a);
 #pragma unroll
             for (int p = 0; p < NQ; ++p) {
                 A[X(p, lm, lm2)] = a[p];
@@ -152,7 +271,7 @@ struct Elem {
             for (int j = 0; j < NM; ++j) {
                 y[j] = A[X(lm, j, lm2)];
             }
-            cols<NQ>(launder<TAG + 2>(Bm), y, b);
+            cols<NQ>(op_b<TAG + 2, EO, NM, NQ>(Bm), y, b);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 B[X(lm, q, lm2)] = b[q];
@@ -165,8 +284,8 @@ struct Elem {
             for (int k = 0; k < NM; ++k) {
                 y[k] = B[X(lq, lq2, k)];
             }
-            cols<NQ>(launder<TAG + 3>(Bm), y, U);
-            cols<NQ>(launder<TAG + 4>(Dm), U, qt);
+            cols<NQ>(op_b<TAG + 3, EO, NM, NQ>(Bm), y, U);
+            cols<NQ>(op_d<TAG + 4, EO, NQ>(Dm), U, qt);
 #pragma unroll
             for (int r = 0; r < NQ; ++r) {
                 A[X(lq, lq2, r)] = U[r];
@@ -181,7 +300,7 @@ struct Elem {
                 y[q] = A[X(lq, q, lq2)];
                 z[q] = A[X(q, lq, lq2)];
             }
-            cols2<NQ>(launder<TAG + 5>(Dm), y, qs, z, qr);
+            cols2<NQ>(op_d<TAG + 5, EO, NQ>(Dm), y, qs, z, qr);
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 B[X(lq, q, lq2)] = qs[q];
@@ -193,7 +312,7 @@ struct Elem {
     // rr (lane (q,r)), rs in B, rt in C -> o (lane (j,k), l < nm^2).
     // Starts after a workgroup synchronisation (the caller's writes of B, C);
     // ends with the workgroup synchronised.
-    template <int TAG>
+    template <int TAG, bool EO = false>
     __device__ void div(const double* Bm, const double* Dm, const double (&rr)[NQ], double (&o)[NM])
     {
         double Ws[NQ], Wt[NQ];
@@ -204,7 +323,7 @@ struct Elem {
                 y[q] = B[X(lq, q, lq2)];
                 z[q] = C[X(lq, lq2, q)];
             }
-            rows3<NQ>(launder<TAG + 1>(Dm), rr, Wr, y, Ws, z, Wt);
+            rows3<NQ>(op_d<TAG + 1, EO, NQ>(Dm), rr, Wr, y, Ws, z, Wt);
 #pragma unroll
             for (int p = 0; p < NQ; ++p) {
                 A[X(p, lq, lq2)] = Wr[p];
@@ -225,7 +344,7 @@ struct Elem {
                 const int x = X(lq, lq2, r);
                 W[r] = Wt[r] + A[x] + B[x];
             }
-            rows<NQ>(launder<TAG + 2>(Bm), W, Y);
+            rows<NQ>(op_b<TAG + 2, EO, NM, NQ>(Bm), W, Y);
 #pragma unroll
             for (int k = 0; k < NM; ++k) {
                 A[X(lq, lq2, k)] = Y[k];
@@ -238,7 +357,7 @@ struct Elem {
             for (int q = 0; q < NQ; ++q) {
                 y[q] = A[X(lm, q, lm2)];
             }
-            rows<NQ>(launder<TAG + 3>(Bm), y, Z);
+            rows<NQ>(op_b<TAG + 3, EO, NM, NQ>(Bm), y, Z);
 #pragma unroll
             for (int j = 0; j < NM; ++j) {
                 B[X(lm, j, lm2)] = Z[j];
@@ -251,7 +370,7 @@ struct Elem {
             for (int p = 0; p < NQ; ++p) {
                 z[p] = B[X(p, lm, lm2)];
             }
-            rows<NQ>(launder<TAG + 4>(Bm), z, o);
+            rows<NQ>(op_b<TAG + 4, EO, NM, NQ>(Bm), z, o);
         }
         __syncthreads();
     }

@@ -6,7 +6,7 @@
 
 #include <cmath>
 
-#include "bk13t_body.hpp"
+#include "bkx_eo.hpp"
 #include "bkx.h"
 #include "bkx_launch.hpp"
 
@@ -51,6 +51,10 @@ int bk3_apply_red(int form, int variant, int p, size_t nelmt, const double* B, c
                   const void* u, double* v, hipStream_t s);
 int bk3t_apply_eo_f64(int variant, int p, size_t nelmt, const double* P, const double* G, const double* u, double* v,
                       hipStream_t s);
+int bk3_adjoint_eo_f64(int p, size_t nelmt, const double* P, const double* G, const double* kappa, const double* u,
+                       const double* b, double* dldu, double* dldk, double* loss, hipStream_t s);
+int bk3_adjoint_eo_f32w(int p, size_t nelmt, const double* P, const float* G, const float* kappa, const float* u,
+                        const float* b, double* dldu, double* dldk, double* loss, hipStream_t s);
 int bk3_apply_f32w_eo(int variant, int p, size_t nelmt, const double* P, const float* G, const float* u, double* v,
                       hipStream_t s);
 }  // namespace bkx
@@ -117,12 +121,12 @@ bool tune_index_ok(int op, int form, int p)
 
 int variant_of(int op, int form, int p) { return g_variant[op][form][p].load(std::memory_order_relaxed); }
 
-// The even-odd form (bkx::bk13t::EoMat) of the row-major R x C matrix M,
+// The even-odd form (bkx::EoMat) of the row-major R x C matrix M,
 // after checking M[R-1-a][C-1-b] = S M[a][b] to 1e-14 of max |M|.
 template <int R, int C, int S>
 bool eo_form(const double* M, double* out)
 {
-    using E = bkx::bk13t::EoMat<R, C, S>;
+    using E = bkx::EoMat<R, C, S>;
     double big = 0.0;
     for (int x = 0; x < R * C; ++x) {
         big = std::fmax(big, std::fabs(M[x]));
@@ -343,7 +347,7 @@ size_t bkx_bk3_basis_size(int p)
     }
     return size_t(bkx::with_order<1, 8>(p, [](auto O) {
         constexpr int NM = decltype(O)::value + 1;
-        return bkx::bk13t::EoMat<NM, NM + 1, 1>::size + bkx::bk13t::EoMat<NM + 1, NM + 1, -1>::size;
+        return bkx::EoMat<NM, NM + 1, 1>::size + bkx::EoMat<NM + 1, NM + 1, -1>::size;
     }));
 }
 
@@ -359,7 +363,7 @@ int bkx_bk3_basis_prepare(int p, const double* basis, const double* dbasis, doub
         constexpr int NM = decltype(O)::value + 1;
         constexpr int NQ = NM + 1;
         const bool ok = eo_form<NM, NQ, 1>(basis, prepared) &&
-                        eo_form<NQ, NQ, -1>(dbasis, prepared + bkx::bk13t::EoMat<NM, NQ, 1>::size);
+                        eo_form<NQ, NQ, -1>(dbasis, prepared + bkx::EoMat<NM, NQ, 1>::size);
         return ok ? int(BKX_SUCCESS) : int(BKX_ERROR_ARG);
     });
 }
@@ -495,6 +499,38 @@ int bkx_bk3_adjoint_f64(int p, size_t nelmt, const double* basis, const double* 
         return BKX_ERROR_NULL;
     }
     return bkx::bk3_adjoint_f64(p, nelmt, basis, dbasis, G, kappa, u, b, dldu, dldkappa, loss, as_stream(stream));
+}
+
+int bkx_bk3_adjoint_prepared_f64(int p, size_t nelmt, const double* prepared, const double* G, const double* kappa,
+                                 const double* u, const double* b, double* dldu, double* dldkappa, double* loss,
+                                 bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!prepared || !G || !kappa || !u || !b || !dldu || !dldkappa || !loss) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_adjoint_eo_f64(p, nelmt, prepared, G, kappa, u, b, dldu, dldkappa, loss, as_stream(stream));
+}
+
+int bkx_bk3_adjoint_prepared_f32w(int p, size_t nelmt, const double* prepared, const float* G, const float* kappa,
+                                  const float* u, const float* b, double* dldu, double* dldkappa, double* loss,
+                                  bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!prepared || !G || !kappa || !u || !b || !dldu || !dldkappa || !loss) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_adjoint_eo_f32w(p, nelmt, prepared, G, kappa, u, b, dldu, dldkappa, loss, as_stream(stream));
 }
 
 int bkx_bk3_vjp_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,

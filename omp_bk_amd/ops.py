@@ -249,6 +249,33 @@ def bk3_adjoint(p, basis, dbasis, G, kappa, u, b, form="f64"):
     return L, dldu, dldk, loss_e
 
 
+def bk3_adjoint_prepared(p, prepared, G, kappa, u, b, form="f64"):
+    """bk3_adjoint with the bases in the even-odd form of bk3_basis; form
+    "f64" or "f32w".  The results agree with bk3_adjoint's to rounding."""
+    _order_ok(3, p)
+    if form not in ("f64", "f32w"):
+        raise ValueError("form must be f64 or f32w")
+    sdt = torch.float64 if form == "f64" else torch.float32
+    nm, nq = p + 1, p + 2
+    if u.numel() % nm ** 3:
+        raise ValueError("u must hold whole elements of %d values" % nm ** 3)
+    ne = u.numel() // nm ** 3
+    _check("prepared", prepared, _lib.lib().bkx_bk3_basis_size(p))
+    _check("G", G, ne * 6 * nq ** 3, sdt)
+    _check("kappa", kappa, ne * nq ** 3, sdt)
+    _check("u", u, ne * nm ** 3, sdt)
+    _check("b", b, ne * nm ** 3, sdt)
+    dldu = torch.empty(u.shape, dtype=torch.float64, device=u.device)
+    dldk = torch.empty(kappa.shape, dtype=torch.float64, device=u.device)
+    loss_e = torch.empty(ne, dtype=torch.float64, device=u.device)
+    L = torch.empty((), dtype=torch.float64, device=u.device)
+    with torch.cuda.device(u.device):
+        _lib.call("bkx_bk3_adjoint_prepared_" + form, p, ne, prepared.data_ptr(), G.data_ptr(), kappa.data_ptr(),
+                  u.data_ptr(), b.data_ptr(), dldu.data_ptr(), dldk.data_ptr(), loss_e.data_ptr(), _stream(u))
+        _lib.call("bkx_sum_f64", loss_e.data_ptr(), ne, L.data_ptr(), _stream(u))
+    return L, dldu, dldk, loss_e
+
+
 def bk3_vjp(p, basis, dbasis, G, kappa, u, w, form="f64"):
     """Vector-Jacobian product of v = A(kappa) u with cotangent w:
     (A(kappa) w, (grad w) . G (grad u)).  form as for bk3_kappa (w in the

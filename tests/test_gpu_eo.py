@@ -87,3 +87,30 @@ def test_prepared_launch_shapes(form, variants, p):
 def test_bases_that_are_not_centro_symmetric_are_refused():
     with pytest.raises(ValueError):
         ops.bk3_basis(3, O.test_basis(4, 5), O.test_basis(5, 5), device=DEV)
+
+
+@pytest.mark.parametrize("form", ["f64", "f32w"])
+@pytest.mark.parametrize("p", [1, 3, 6, 8])
+def test_prepared_adjoint_against_oracle_and_direct(form, p):
+    """The fused adjoint (dL/du, dL/dkappa and L) with the bases in even-odd
+    form: against the oracle's adjoint and the direct kernel."""
+    nm, nq = p + 1, p + 2
+    E = 301
+    B, D = geo.bk13_bases(p)
+    rng = np.random.default_rng(600 + p)
+    u, G = O.fill_random(E * nm ** 3, E * 6 * nq ** 3, seed=700 + p)
+    kappa = np.exp(0.5 * rng.standard_normal(E * nq ** 3))
+    b = rng.uniform(-1, 1, E * nm ** 3)
+    dt = torch.float64 if form == "f64" else torch.float32
+    if form == "f32w":
+        u, G, kappa, b = (x.astype(np.float32).astype(np.float64) for x in (u, G, kappa, b))
+    P = ops.bk3_basis(p, _t(B), _t(D))
+    print("prepared adjoint %s p=%d E=%d" % (form, p, E), flush=True)
+    L, dldu, dldk, _ = ops.bk3_adjoint_prepared(p, P, _t(G, dt), _t(kappa, dt), _t(u, dt), _t(b, dt), form=form)
+    Ld, dldu_d, dldk_d, _ = ops.bk3_adjoint(p, _t(B), _t(D), _t(G, dt), _t(kappa, dt), _t(u, dt), _t(b, dt),
+                                            form=form)
+    loss, _, ref_u, ref_k = O.bk3_adjoint(nq, B, D, G, kappa, u, b)
+    got_u, got_k = dldu.cpu().numpy(), dldk.cpu().numpy()
+    assert _rel(got_u, ref_u) <= 1e-13 and _rel(got_k, ref_k) <= 1e-13
+    assert abs(float(L.item()) - loss) <= 1e-13 * abs(loss)
+    assert _rel(got_u, dldu_d.cpu().numpy()) <= 1e-14 and _rel(got_k, dldk_d.cpu().numpy()) <= 1e-14
