@@ -418,6 +418,8 @@ constexpr int bk13t_nqv(bool bk3, int gpf, int nq) { return bk3 ? (gpf > 0 ? gpf
 // valid column, clamped) and the prologue loads G before u, as the phases
 // do, so that the memory operations in flight at the top of the loop are the
 // same on both paths into it and the compiler's wait counts stay exact.
+// With EO (FP64 arithmetic) B and D point at their even-odd forms (EoMat),
+// and every contraction runs in that form; the phases are unchanged.
 template <int NM, int NQ, bool BK3, int DIV, int GPF, int FORM, int NT, bool PIPE>
 __device__ __forceinline__ void bk13t_load(const typename Form<FORM>::TS* __restrict__ Qe,
                                            const typename Form<FORM>::TS* __restrict__ ue, int slot, int l,
