@@ -73,13 +73,14 @@ int launch_red(size_t nelmt, const double* B, const double* D, const void* Q, co
 // boxes, BK3 with two rows of G loaded ahead), 2 all (two boxes, BK3 one row
 // ahead in P5), 3 all (three boxes); 4-6 are 2, 3 and 0 with non-temporal
 // loads of u and stores of v, 7 is 2 with JxW / G streamed as well, 8 and 9
-// are 6 with three rows of G ahead (all elements or half of them).  F32C
+// are 6 with three rows of G ahead (all elements or half of them), 10 and 11
+// with five and all nq rows ahead.  F32C
 // and FF (bound by the vector ALU): 0 all, 1 half (two boxes), 2 all (three
 // boxes).
 template <int FORM>
 constexpr int red_variants()
 {
-    return FORM == BKX_FORM_F32W ? 10 : 3;
+    return FORM == BKX_FORM_F32W ? 12 : 3;
 }
 
 template <int NM, bool BK3, int FORM>
@@ -108,6 +109,10 @@ int launch_red_variant(int variant, size_t nelmt, const double* B, const double*
             return launch_red<NM, BK3, 1, 2, BK3 ? 3 : 0, FORM, 1>(nelmt, B, D, Q, u, v, s);
         case 9:
             return launch_red<NM, BK3, 2, 2, BK3 ? 3 : 0, FORM, 1>(nelmt, B, D, Q, u, v, s);
+        case 10:
+            return launch_red<NM, BK3, 1, 2, BK3 ? (NM + 1 < 5 ? NM + 1 : 5) : 0, FORM, 1>(nelmt, B, D, Q, u, v, s);
+        case 11:
+            return launch_red<NM, BK3, 1, 2, BK3 ? NM + 1 : 0, FORM, 1>(nelmt, B, D, Q, u, v, s);
         default:
             return BKX_ERROR_ARG;
         }
@@ -161,8 +166,7 @@ int bk13_red_variants(int form)
 }
 
 // BK3 f32w with the bases in even-odd form (P: B's form, then D's): the
-// launch shapes of the default variants 4, 6, 8 and 9; any other variant
-// runs as 8.
+// launch shapes of the variants 4, 6 and 8-11; any other variant runs as 8.
 int bk3_apply_f32w_eo(int variant, int p, size_t nelmt, const double* P, const float* G, const float* u, double* v,
                       hipStream_t s)
 {
@@ -178,6 +182,10 @@ int bk3_apply_f32w_eo(int variant, int p, size_t nelmt, const double* P, const f
             return launch_red<NM, true, 1, 2, 2, F, 1, true>(nelmt, B, D, G, u, v, s);
         case 9:
             return launch_red<NM, true, 2, 2, 3, F, 1, true>(nelmt, B, D, G, u, v, s);
+        case 10:
+            return launch_red<NM, true, 1, 2, (NM + 1 < 5 ? NM + 1 : 5), F, 1, true>(nelmt, B, D, G, u, v, s);
+        case 11:
+            return launch_red<NM, true, 1, 2, NM + 1, F, 1, true>(nelmt, B, D, G, u, v, s);
         default:
             return launch_red<NM, true, 1, 2, 3, F, 1, true>(nelmt, B, D, G, u, v, s);
         }

@@ -91,7 +91,7 @@ struct VariantTable {
         static const int bk5_f32w[kOrders] = {0, 10, 16, 35, 27, 39, 22, 41, 0};
         static const int bk3_f64[kOrders] = {0, 16, 14, 16, 17, 16, 13, 16, 18};
         static const int bk1_f64[kOrders] = {0, 5, 1, 19, 18, 19, 18, 19, 18};
-        static const int bk3_f32w[kOrders] = {0, 9, 6, 8, 8, 8, 4, 8, 8};
+        static const int bk3_f32w[kOrders] = {0, 9, 6, 8, 8, 10, 4, 8, 8};
         static const int bk3_f32c[kOrders] = {0, 2, 2, 2, 0, 2, 1, 0, 0};
         static const int bk3_ff[kOrders] = {0, 2, 2, 2, 2, 2, 0, 0, 1};
         static const int bk1_f32w[kOrders] = {0, 9, 4, 9, 4, 4, 4, 4, 4};
