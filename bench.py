@@ -335,8 +335,10 @@ def adjoint_config(steps, warmup, p=6, E=16384):
 
 def mixed_batch(steps, warmup, nelmt=200000):
     """BK3 over 200,000 elements of mixed order (Zipf(1.2) over p = 1..8):
-    the per-order tuned single launch and the per-order launches, both
-    replayed from hipGraphs."""
+    the single launch with divisors tuned per order (one workgroup per
+    tile), the per-order launches, and the single launch as a persistent
+    loop over the tiles (one workgroup per CU slot), each replayed from a
+    hipGraph."""
     import numpy as np
     import torch
     from omp_bk_amd import mixed
@@ -352,8 +354,9 @@ def mixed_batch(steps, warmup, nelmt=200000):
     divp, _ = mixed.tune(order, bases, G, u, v, steps=5)
     nbytes = 8 * (2 * plan0.ndof + plan0.ngeo)
     out = {"nelmt": nelmt, "counts_by_p": np.bincount(order, minlength=9)[1:].tolist(), "bytes": nbytes}
-    for name, by_order, div in (("single_launch_graph", False, div1), ("per_order_graph", True, divp)):
-        plan = mixed.MixedPlan(order, div_by_order=div)
+    for name, by_order, div, grid in (("single_launch_graph", False, div1, 0), ("per_order_graph", True, divp, 0),
+                                      ("persistent_graph", False, div1, plan0.grid)):
+        plan = mixed.MixedPlan(order, div_by_order=div, grid=grid)
         g = plan.graph(bases, G, u, v, by_order=by_order)
         ms, _ = time_steps(g.launch, steps, warmup)
         g.close()

@@ -315,8 +315,8 @@ int bkx_bk3_mixed_apply_by_order_f64(const bkx_mixed_plan* plan, const double* b
 /* The launch of one order's elements only (used by the tuner). */
 int bkx_bk3_mixed_apply_order_f64(const bkx_mixed_plan* plan, int p, const double* bases, const double* G,
                                   const double* in, double* out, bkx_stream_t stream);
-/* The persistent launch (by_order = 0) or the per-order launches captured
- * in a hipGraph for the given arrays; replay with bkx_graph_launch. */
+/* The single launch (by_order = 0) or the per-order launches captured in a
+ * hipGraph for the given arrays; replay with bkx_graph_launch. */
 int bkx_bk3_mixed_graph_create(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
                                double* out, int by_order, bkx_graph** graph);
 int bkx_graph_launch(bkx_graph* graph, bkx_stream_t stream);
