@@ -62,7 +62,7 @@ def test_prepared_f32w_against_oracle_and_direct(p):
     assert _rel(got, direct) <= 1e-14
 
 
-@pytest.mark.parametrize("form,variants", [("f64", (13, 14, 16, 17, 18, 0)), ("f32w", (4, 6, 8, 9, 0))])
+@pytest.mark.parametrize("form,variants", [("f64", (13, 14, 16, 17, 18, 0)), ("f32w", (4, 6, 8, 9, 10, 11, 0))])
 @pytest.mark.parametrize("p", [2, 5, 7])
 def test_prepared_launch_shapes(form, variants, p):
     L = _lib.lib()
