@@ -338,7 +338,7 @@ def mixed_batch(steps, warmup, nelmt=200000):
     the single launch with divisors tuned per order (one workgroup per
     tile), the per-order launches, and the single launch as a persistent
     loop over the tiles (one workgroup per CU slot), each replayed from a
-    hipGraph."""
+    hipGraph; the first two also with the bases in even-odd form (_eo)."""
     import numpy as np
     import torch
     from omp_bk_amd import mixed
@@ -354,10 +354,14 @@ def mixed_batch(steps, warmup, nelmt=200000):
     divp, _ = mixed.tune(order, bases, G, u, v, steps=5)
     nbytes = 8 * (2 * plan0.ndof + plan0.ngeo)
     out = {"nelmt": nelmt, "counts_by_p": np.bincount(order, minlength=9)[1:].tolist(), "bytes": nbytes}
-    for name, by_order, div, grid in (("single_launch_graph", False, div1, 0), ("per_order_graph", True, divp, 0),
-                                      ("persistent_graph", False, div1, plan0.grid)):
+    P = mixed.prepared_bases(bases)
+    for name, by_order, div, grid, prep in (("single_launch_graph", False, div1, 0, False),
+                                            ("per_order_graph", True, divp, 0, False),
+                                            ("persistent_graph", False, div1, plan0.grid, False),
+                                            ("single_launch_graph_eo", False, div1, 0, True),
+                                            ("per_order_graph_eo", True, divp, 0, True)):
         plan = mixed.MixedPlan(order, div_by_order=div, grid=grid)
-        g = plan.graph(bases, G, u, v, by_order=by_order)
+        g = plan.graph(P if prep else bases, G, u, v, by_order=by_order, prepared=prep)
         ms, _ = time_steps(g.launch, steps, warmup)
         g.close()
         out[name] = {"ms": round(ms, 4), "GDoF/s": round(plan.ndof / (ms * 1e-3) * 1e-9, 3),

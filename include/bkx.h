@@ -319,6 +319,19 @@ int bkx_bk3_mixed_apply_order_f64(const bkx_mixed_plan* plan, int p, const doubl
  * hipGraph for the given arrays; replay with bkx_graph_launch. */
 int bkx_bk3_mixed_graph_create(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
                                double* out, int by_order, bkx_graph** graph);
+/* The same with the bases of every order in even-odd form (see
+ * bkx_bk3_basis_prepare): bkx_bk3_mixed_prepare builds the forms of the
+ * packed HOST bases (layout as above) into the HOST array prepared of
+ * bkx_bk3_mixed_prepared_size() doubles, BKX_ERROR_ARG if an order's bases
+ * are not centro-symmetric; the caller copies it to the device. */
+size_t bkx_bk3_mixed_prepared_size(void);
+int bkx_bk3_mixed_prepare(const double* bases, double* prepared);
+int bkx_bk3_mixed_apply_prepared_f64(const bkx_mixed_plan* plan, const double* prepared, const double* G,
+                                     const double* in, double* out, bkx_stream_t stream);
+int bkx_bk3_mixed_apply_by_order_prepared_f64(const bkx_mixed_plan* plan, const double* prepared, const double* G,
+                                              const double* in, double* out, bkx_stream_t stream);
+int bkx_bk3_mixed_graph_create_prepared(const bkx_mixed_plan* plan, const double* prepared, const double* G,
+                                        const double* in, double* out, int by_order, bkx_graph** graph);
 int bkx_graph_launch(bkx_graph* graph, bkx_stream_t stream);
 int bkx_graph_destroy(bkx_graph* graph);
 

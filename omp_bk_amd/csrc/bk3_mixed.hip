@@ -65,7 +65,8 @@ constexpr int max_smem()
 
 constexpr int kMixedSmem = max_smem<8>();   // doubles
 
-template <int P, int DIV, bool LOOP>
+// EO: bases holds the even-odd forms of B_p and D_p (bkx_eo.hpp), at boff[p].
+template <int P, int DIV, bool LOOP, bool EO>
 __device__ __forceinline__ void run_tile(int tid, const MixedTile& t, const int64_t* __restrict__ uoff,
                                          const int64_t* __restrict__ goff,
                                          const double* __restrict__ bases, const int64_t* __restrict__ boff,
@@ -76,56 +77,56 @@ __device__ __forceinline__ void run_tile(int tid, const MixedTile& t, const int6
     constexpr int NQ = P + 2;
     using S = bk13t::Bk13tShape<NM, NQ, DIV>;
     const double* B = bases + boff[P];
-    const double* D = B + NM * NQ;
+    const double* D = B + (EO ? EoMat<NM, NQ, 1>::size : NM * NQ);
     const int slot = tid / S::nq2;
     const int l = tid - slot * S::nq2;
     if constexpr (!LOOP) {
         // one round: the tile holds at most epb elements
         const bool active = slot < S::epb && slot < t.count;
         const int e = t.first + (active ? slot : 0);
-        bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf>(B, D, G + goff[e], u + uoff[e],
-                                                                            v + uoff[e], active, slot, l, smem);
+        bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf, BKX_FORM_F64, double, 0, EO>(
+            B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l, smem);
     } else {
         for (int r0 = 0; r0 < t.count; r0 += S::epb) {
             const int idx = r0 + slot;
             const bool active = slot < S::epb && idx < t.count;
             const int e = t.first + (active ? idx : 0);
-            bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf>(
+            bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf, BKX_FORM_F64, double, 0, EO>(
                 B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l, smem, t.first + r0);
             __syncthreads();
         }
     }
 }
 
-template <int DIV, bool LOOP>
+template <int DIV, bool LOOP, bool EO>
 __device__ __forceinline__ void run_tile_p(int tid, const MixedTile& t, const int64_t* uoff, const int64_t* goff,
                                            const double* bases, const int64_t* boff,
                                            const double* G, const double* u, double* v, double* smem)
 {
     switch (t.p) {
     case 1:
-        run_tile<1, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<1, DIV, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 2:
-        run_tile<2, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<2, DIV, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 3:
-        run_tile<3, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<3, DIV, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 4:
-        run_tile<4, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<4, DIV, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 5:
-        run_tile<5, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<5, DIV, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 6:
-        run_tile<6, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<6, DIV, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     case 7:
-        run_tile<7, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<7, DIV, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     default:
-        run_tile<8, DIV, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+        run_tile<8, DIV, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         break;
     }
 }
@@ -136,7 +137,7 @@ __device__ __forceinline__ void run_tile_p(int tid, const MixedTile& t, const in
 // In the persistent form the basis rows are read with vector loads (loads
 // after the loop's barriers and stores cannot be proven unclobbered, so they
 // do not become scalar loads), which costs registers and occupancy.
-template <bool LOOP>
+template <bool LOOP, bool EO>
 __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
     int ntiles, const MixedTile* __restrict__ tiles, const int64_t* __restrict__ uoff,
     const int64_t* __restrict__ goff, const double* __restrict__ bases,
@@ -147,9 +148,9 @@ __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
     auto one = [&](int k, int tid) {
         const MixedTile t = tiles[k];
         if (t.div == 2) {
-            run_tile_p<2, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+            run_tile_p<2, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         } else {
-            run_tile_p<1, LOOP>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
+            run_tile_p<1, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         }
     };
     if constexpr (LOOP) {
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
 constexpr int kOrderGpf = 3;
 constexpr int kOrderNt = 1;
 
-template <int P, int DIV>
+template <int P, int DIV, bool EO>
 __global__ __launch_bounds__((bk13t::Bk13tShape<P + 1, P + 2, DIV>::threads))
 __attribute__((amdgpu_waves_per_eu(4))) void bk3_order_kernel(
     int first, int count, const int64_t* __restrict__ uoff, const int64_t* __restrict__ goff,
@@ -194,7 +195,7 @@ __attribute__((amdgpu_waves_per_eu(4))) void bk3_order_kernel(
     const int idx = int(blockIdx.x) * S::epb + slot;
     const bool active = slot < S::epb && idx < count;
     const int e = first + (active ? idx : 0);
-    bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kOrderGpf, BKX_FORM_F64, double, kOrderNt>(
+    bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kOrderGpf, BKX_FORM_F64, double, kOrderNt, EO>(
         B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l, smem);
 }
 
@@ -214,11 +215,13 @@ struct bkx_mixed_plan {
     int elem_begin[10] = {};            // per order: range of the order-sorted element list
     int div_p[10] = {};                 // per order: elements-per-workgroup divisor
     int64_t boff_h[10] = {};            // per order: offset of B_p in the bases array
+    int64_t eoff_h[10] = {};            // per order: offset of B_p's even-odd form in the prepared array
     std::vector<int64_t> uoff_h, goff_h;
     bkx::MixedTile* tiles_persist = nullptr;
     int64_t* uoff = nullptr;    // by position in the order-sorted element list
     int64_t* goff = nullptr;
     int64_t* boff = nullptr;
+    int64_t* eoff = nullptr;
 };
 
 struct bkx_graph {
@@ -241,36 +244,39 @@ int upload(T** dst, const T* src, size_t n)
     return BKX_SUCCESS;
 }
 
+template <bool EO>
 int launch_tiles(const bkx::MixedTile* tiles, int ntiles, bool persistent, const bkx_mixed_plan* pl,
                  const double* bases, const double* G, const double* in, double* out, hipStream_t s)
 {
     if (ntiles == 0) {
         return BKX_SUCCESS;
     }
+    const int64_t* off = EO ? pl->eoff : pl->boff;
     if (persistent) {
-        hipLaunchKernelGGL(bkx::bk3_mixed_kernel<true>, dim3(std::min(pl->grid, ntiles)), dim3(bkx::kMixedThreads), 0,
-                           s, ntiles, tiles, pl->uoff, pl->goff, bases, pl->boff, G, in, out);
+        hipLaunchKernelGGL((bkx::bk3_mixed_kernel<true, EO>), dim3(std::min(pl->grid, ntiles)), dim3(bkx::kMixedThreads),
+                           0, s, ntiles, tiles, pl->uoff, pl->goff, bases, off, G, in, out);
     } else {
-        hipLaunchKernelGGL(bkx::bk3_mixed_kernel<false>, dim3(ntiles), dim3(bkx::kMixedThreads), 0, s, ntiles, tiles,
-                           pl->uoff, pl->goff, bases, pl->boff, G, in, out);
+        hipLaunchKernelGGL((bkx::bk3_mixed_kernel<false, EO>), dim3(ntiles), dim3(bkx::kMixedThreads), 0, s, ntiles,
+                           tiles, pl->uoff, pl->goff, bases, off, G, in, out);
     }
     return bkx::launch_status();
 }
 
-template <int P, int DIV>
+template <int P, int DIV, bool EO>
 void launch_order_div(int first, int count, const bkx_mixed_plan* pl, const double* bases, const double* G,
                       const double* in, double* out, hipStream_t s)
 {
     using S = bkx::bk13t::Bk13tShape<P + 1, P + 2, DIV>;
-    const double* B = bases + pl->boff_h[P];
-    const double* D = B + (P + 1) * (P + 2);
+    const double* B = bases + (EO ? pl->eoff_h[P] : pl->boff_h[P]);
+    const double* D = B + (EO ? bkx::EoMat<P + 1, P + 2, 1>::size : (P + 1) * (P + 2));
     const unsigned grid = unsigned((count + S::epb - 1) / S::epb);
-    hipLaunchKernelGGL((bkx::bk3_order_kernel<P, DIV>), dim3(grid), dim3(S::threads), 0, s, first, count, pl->uoff,
-                       pl->goff, B, D, G, in, out);
+    hipLaunchKernelGGL((bkx::bk3_order_kernel<P, DIV, EO>), dim3(grid), dim3(S::threads), 0, s, first, count,
+                       pl->uoff, pl->goff, B, D, G, in, out);
 }
 
 // One order's elements, by that order's own kernel: workgroup k takes
 // elements k epb .. k epb + epb - 1 of the order's part of the sorted list.
+template <bool EO>
 int launch_order(int p, const bkx_mixed_plan* pl, const double* bases, const double* G, const double* in,
                  double* out, hipStream_t s)
 {
@@ -283,12 +289,93 @@ int launch_order(int p, const bkx_mixed_plan* pl, const double* bases, const dou
     return bkx::with_order<1, 8>(p, [&](auto P) {
         constexpr int PP = decltype(P)::value;
         if (half) {
-            launch_order_div<PP, 2>(first, count, pl, bases, G, in, out, s);
+            launch_order_div<PP, 2, EO>(first, count, pl, bases, G, in, out, s);
         } else {
-            launch_order_div<PP, 1>(first, count, pl, bases, G, in, out, s);
+            launch_order_div<PP, 1, EO>(first, count, pl, bases, G, in, out, s);
         }
         return bkx::launch_status();
     });
+}
+
+// The entry points' bodies, with the bases (EO: the prepared even-odd
+// forms, bkx_bk3_mixed_prepare) as given.
+template <bool EO>
+int apply_single(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in, double* out,
+                 hipStream_t s)
+{
+    if (!plan) {
+        return BKX_ERROR_NULL;
+    }
+    if (plan->nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!bases || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return launch_tiles<EO>(plan->tiles_persist, plan->ntiles, plan->persistent, plan, bases, G, in, out, s);
+}
+
+template <bool EO>
+int apply_by_order(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in, double* out,
+                   hipStream_t s)
+{
+    if (!plan) {
+        return BKX_ERROR_NULL;
+    }
+    if (plan->nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!bases || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    for (int p = 1; p <= 8; ++p) {
+        const int st = launch_order<EO>(p, plan, bases, G, in, out, s);
+        if (st != BKX_SUCCESS) {
+            return st;
+        }
+    }
+    return BKX_SUCCESS;
+}
+
+template <bool EO>
+int graph_create(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in, double* out,
+                 int by_order, bkx_graph** graph)
+{
+    if (!plan || !graph) {
+        return BKX_ERROR_NULL;
+    }
+    *graph = nullptr;
+    auto* g = new (std::nothrow) bkx_graph;
+    if (!g) {
+        return BKX_ERROR_DEVICE;
+    }
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
+        delete g;
+        return BKX_ERROR_DEVICE;
+    }
+    int st = BKX_SUCCESS;
+    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        st = BKX_ERROR_DEVICE;
+    } else {
+        st = by_order ? apply_by_order<EO>(plan, bases, G, in, out, s) : apply_single<EO>(plan, bases, G, in, out, s);
+        hipGraph_t captured = nullptr;
+        const hipError_t end = hipStreamEndCapture(s, &captured);
+        if (st == BKX_SUCCESS && end != hipSuccess) {
+            st = BKX_ERROR_DEVICE;
+        }
+        g->graph = captured;
+        if (st == BKX_SUCCESS && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
+            st = BKX_ERROR_DEVICE;
+        }
+    }
+    (void)hipStreamDestroy(s);
+    if (st != BKX_SUCCESS) {
+        bkx_graph_destroy(g);
+        return st;
+    }
+    *graph = g;
+    return BKX_SUCCESS;
 }
 
 }  // namespace
@@ -378,6 +465,14 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
     }
     boff[9] = bo;
     std::copy(boff, boff + 10, pl->boff_h);
+    int64_t eoff[10] = {};
+    int64_t eo = 0;
+    for (int p = 1; p <= 8; ++p) {
+        eoff[p] = eo;
+        eo += int64_t(bkx_bk3_basis_size(p));
+    }
+    eoff[9] = eo;
+    std::copy(eoff, eoff + 10, pl->eoff_h);
 
     const bool grid_given = grid > 0;
     if (!grid_given) {
@@ -385,7 +480,7 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
         if (hipGetDevice(&dev) == hipSuccess) {
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         }
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bkx::bk3_mixed_kernel<true>, bkx::kMixedThreads,
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bkx::bk3_mixed_kernel<true, false>, bkx::kMixedThreads,
                                                          0) !=
                 hipSuccess ||
             per_cu < 1) {
@@ -408,6 +503,7 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
     st = st ? st : upload(&pl->uoff, uoff_s.data(), nelmt);
     st = st ? st : upload(&pl->goff, goff_s.data(), nelmt);
     st = st ? st : upload(&pl->boff, boff, 10);
+    st = st ? st : upload(&pl->eoff, eoff, 10);
     if (st != BKX_SUCCESS) {
         bkx_mixed_plan_destroy(pl);
         return st;
@@ -423,7 +519,8 @@ int bkx_mixed_plan_destroy(bkx_mixed_plan* plan)
     }
     for (void* ptr : {static_cast<void*>(plan->tiles_persist),
                       static_cast<void*>(plan->uoff),
-                      static_cast<void*>(plan->goff), static_cast<void*>(plan->boff)}) {
+                      static_cast<void*>(plan->goff), static_cast<void*>(plan->boff),
+                      static_cast<void*>(plan->eoff)}) {
         if (ptr) {
             (void)hipFree(ptr);
         }
@@ -465,38 +562,13 @@ int bkx_mixed_plan_offsets(const bkx_mixed_plan* plan, int64_t* uoff, int64_t* g
 int bkx_bk3_mixed_apply_f64(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
                             double* out, bkx_stream_t stream)
 {
-    if (!plan) {
-        return BKX_ERROR_NULL;
-    }
-    if (plan->nelmt == 0) {
-        return BKX_SUCCESS;
-    }
-    if (!bases || !G || !in || !out) {
-        return BKX_ERROR_NULL;
-    }
-    return launch_tiles(plan->tiles_persist, plan->ntiles, plan->persistent, plan, bases, G, in, out,
-                        reinterpret_cast<hipStream_t>(stream));
+    return apply_single<false>(plan, bases, G, in, out, reinterpret_cast<hipStream_t>(stream));
 }
 
 int bkx_bk3_mixed_apply_by_order_f64(const bkx_mixed_plan* plan, const double* bases, const double* G,
                                      const double* in, double* out, bkx_stream_t stream)
 {
-    if (!plan) {
-        return BKX_ERROR_NULL;
-    }
-    if (plan->nelmt == 0) {
-        return BKX_SUCCESS;
-    }
-    if (!bases || !G || !in || !out) {
-        return BKX_ERROR_NULL;
-    }
-    for (int p = 1; p <= 8; ++p) {
-        const int st = launch_order(p, plan, bases, G, in, out, reinterpret_cast<hipStream_t>(stream));
-        if (st != BKX_SUCCESS) {
-            return st;
-        }
-    }
-    return BKX_SUCCESS;
+    return apply_by_order<false>(plan, bases, G, in, out, reinterpret_cast<hipStream_t>(stream));
 }
 
 int bkx_bk3_mixed_apply_order_f64(const bkx_mixed_plan* plan, int p, const double* bases, const double* G,
@@ -511,48 +583,59 @@ int bkx_bk3_mixed_apply_order_f64(const bkx_mixed_plan* plan, int p, const doubl
     if (!bases || !G || !in || !out) {
         return BKX_ERROR_NULL;
     }
-    return launch_order(p, plan, bases, G, in, out, reinterpret_cast<hipStream_t>(stream));
+    return launch_order<false>(p, plan, bases, G, in, out, reinterpret_cast<hipStream_t>(stream));
 }
 
 int bkx_bk3_mixed_graph_create(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
                                double* out, int by_order, bkx_graph** graph)
 {
-    if (!plan || !graph) {
+    return graph_create<false>(plan, bases, G, in, out, by_order, graph);
+}
+
+size_t bkx_bk3_mixed_prepared_size(void)
+{
+    size_t n = 0;
+    for (int p = 1; p <= 8; ++p) {
+        n += bkx_bk3_basis_size(p);
+    }
+    return n;
+}
+
+int bkx_bk3_mixed_prepare(const double* bases, double* prepared)
+{
+    if (!bases || !prepared) {
         return BKX_ERROR_NULL;
     }
-    *graph = nullptr;
-    auto* g = new (std::nothrow) bkx_graph;
-    if (!g) {
-        return BKX_ERROR_DEVICE;
-    }
-    hipStream_t s = nullptr;
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) {
-        delete g;
-        return BKX_ERROR_DEVICE;
-    }
-    int st = BKX_SUCCESS;
-    if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        st = BKX_ERROR_DEVICE;
-    } else {
-        st = by_order ? bkx_bk3_mixed_apply_by_order_f64(plan, bases, G, in, out, s)
-                      : bkx_bk3_mixed_apply_f64(plan, bases, G, in, out, s);
-        hipGraph_t captured = nullptr;
-        const hipError_t end = hipStreamEndCapture(s, &captured);
-        if (st == BKX_SUCCESS && end != hipSuccess) {
-            st = BKX_ERROR_DEVICE;
+    const double* b = bases;
+    double* f = prepared;
+    for (int p = 1; p <= 8; ++p) {
+        const int nm = bkx::n_of(p), nq = bkx::q_of(p);
+        const int st = bkx_bk3_basis_prepare(p, b, b + nm * nq, f);
+        if (st != BKX_SUCCESS) {
+            return st;
         }
-        g->graph = captured;
-        if (st == BKX_SUCCESS && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) {
-            st = BKX_ERROR_DEVICE;
-        }
+        b += nm * nq + nq * nq;
+        f += bkx_bk3_basis_size(p);
     }
-    (void)hipStreamDestroy(s);
-    if (st != BKX_SUCCESS) {
-        bkx_graph_destroy(g);
-        return st;
-    }
-    *graph = g;
     return BKX_SUCCESS;
+}
+
+int bkx_bk3_mixed_apply_prepared_f64(const bkx_mixed_plan* plan, const double* prepared, const double* G,
+                                     const double* in, double* out, bkx_stream_t stream)
+{
+    return apply_single<true>(plan, prepared, G, in, out, reinterpret_cast<hipStream_t>(stream));
+}
+
+int bkx_bk3_mixed_apply_by_order_prepared_f64(const bkx_mixed_plan* plan, const double* prepared, const double* G,
+                                              const double* in, double* out, bkx_stream_t stream)
+{
+    return apply_by_order<true>(plan, prepared, G, in, out, reinterpret_cast<hipStream_t>(stream));
+}
+
+int bkx_bk3_mixed_graph_create_prepared(const bkx_mixed_plan* plan, const double* prepared, const double* G,
+                                        const double* in, double* out, int by_order, bkx_graph** graph)
+{
+    return graph_create<true>(plan, prepared, G, in, out, by_order, graph);
 }
 
 int bkx_graph_launch(bkx_graph* graph, bkx_stream_t stream)

@@ -41,11 +41,19 @@ def _bind():
     L.bkx_bk3_mixed_apply_by_order_f64.argtypes = [_vp] + [_vp] * 5
     L.bkx_bk3_mixed_apply_order_f64.argtypes = [_vp, ctypes.c_int] + [_vp] * 5
     L.bkx_bk3_mixed_graph_create.argtypes = [_vp] + [_vp] * 4 + [ctypes.c_int, ctypes.POINTER(_vp)]
+    L.bkx_bk3_mixed_prepared_size.argtypes = []
+    L.bkx_bk3_mixed_prepared_size.restype = ctypes.c_size_t
+    L.bkx_bk3_mixed_prepare.argtypes = [_vp, _vp]
+    L.bkx_bk3_mixed_apply_prepared_f64.argtypes = [_vp] + [_vp] * 5
+    L.bkx_bk3_mixed_apply_by_order_prepared_f64.argtypes = [_vp] + [_vp] * 5
+    L.bkx_bk3_mixed_graph_create_prepared.argtypes = [_vp] + [_vp] * 4 + [ctypes.c_int, ctypes.POINTER(_vp)]
     L.bkx_graph_launch.argtypes = [_vp, _vp]
     L.bkx_graph_destroy.argtypes = [_vp]
     for name in ("bkx_mixed_plan_create", "bkx_mixed_plan_destroy", "bkx_mixed_plan_info", "bkx_mixed_plan_offsets",
                  "bkx_bk3_mixed_apply_f64", "bkx_bk3_mixed_apply_by_order_f64", "bkx_bk3_mixed_apply_order_f64",
-                 "bkx_bk3_mixed_graph_create", "bkx_graph_launch", "bkx_graph_destroy"):
+                 "bkx_bk3_mixed_graph_create", "bkx_graph_launch", "bkx_graph_destroy", "bkx_bk3_mixed_prepare",
+                 "bkx_bk3_mixed_apply_prepared_f64", "bkx_bk3_mixed_apply_by_order_prepared_f64",
+                 "bkx_bk3_mixed_graph_create_prepared"):
         getattr(L, name).restype = ctypes.c_int
     L._bkx_mixed_bound = True
     return L
@@ -82,6 +90,21 @@ def packed_bases():
         B, D = geo.bk13_bases(p)
         parts += [B.reshape(-1), D.reshape(-1)]
     return np.concatenate(parts)
+
+
+def prepared_bases(bases):
+    """The even-odd forms of the packed bases of every order
+    (bkx_bk3_mixed_prepare, on the host), on the device of ``bases``; raises
+    ValueError when an order's bases are not centro-symmetric."""
+    L = _bind()
+    host = np.ascontiguousarray(torch.as_tensor(bases).detach().cpu().numpy(), dtype=np.float64).reshape(-1)
+    out = np.empty(L.bkx_bk3_mixed_prepared_size())
+    st = L.bkx_bk3_mixed_prepare(host.ctypes.data, out.ctypes.data)
+    if st == 5:
+        raise ValueError("the bases are not centro-symmetric")
+    _check("bkx_bk3_mixed_prepare", st)
+    device = bases.device if isinstance(bases, torch.Tensor) else None
+    return torch.as_tensor(out, device=device)
 
 
 def affine_factors(order, goff, ngeo, amp=0.1, seed=7, device=None):
@@ -163,21 +186,25 @@ class MixedPlan:
             pass
 
     def _args(self, bases, G, u, v):
+        # bases: the packed bases, or their even-odd forms (prepared_bases)
         for t in (bases, G, u, v):
             assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
         assert u.numel() == self.ndof and v.numel() == self.ndof and G.numel() == self.ngeo
         return bases.data_ptr(), G.data_ptr(), u.data_ptr(), v.data_ptr()
 
-    def apply(self, bases, G, u, v):
-        """One launch over all tiles, on the current torch stream."""
-        _check("bkx_bk3_mixed_apply_f64", _bind().bkx_bk3_mixed_apply_f64(
-            self.handle, *self._args(bases, G, u, v), torch.cuda.current_stream().cuda_stream))
+    def apply(self, bases, G, u, v, prepared=False):
+        """One launch over all tiles, on the current torch stream; with
+        prepared, bases are the even-odd forms of prepared_bases."""
+        name = "bkx_bk3_mixed_apply_prepared_f64" if prepared else "bkx_bk3_mixed_apply_f64"
+        _check(name, getattr(_bind(), name)(self.handle, *self._args(bases, G, u, v),
+                                            torch.cuda.current_stream().cuda_stream))
         return v
 
-    def apply_by_order(self, bases, G, u, v):
+    def apply_by_order(self, bases, G, u, v, prepared=False):
         """One launch per order."""
-        _check("bkx_bk3_mixed_apply_by_order_f64", _bind().bkx_bk3_mixed_apply_by_order_f64(
-            self.handle, *self._args(bases, G, u, v), torch.cuda.current_stream().cuda_stream))
+        name = "bkx_bk3_mixed_apply_by_order_prepared_f64" if prepared else "bkx_bk3_mixed_apply_by_order_f64"
+        _check(name, getattr(_bind(), name)(self.handle, *self._args(bases, G, u, v),
+                                            torch.cuda.current_stream().cuda_stream))
         return v
 
     def apply_order(self, p, bases, G, u, v):
@@ -185,16 +212,17 @@ class MixedPlan:
             self.handle, p, *self._args(bases, G, u, v), torch.cuda.current_stream().cuda_stream))
         return v
 
-    def graph(self, bases, G, u, v, by_order=False):
+    def graph(self, bases, G, u, v, by_order=False, prepared=False):
         """A hipGraph of the launch(es) for these arrays (MixedGraph)."""
-        return MixedGraph(self, bases, G, u, v, by_order)
+        return MixedGraph(self, bases, G, u, v, by_order, prepared)
 
 
 class MixedGraph:
-    def __init__(self, plan, bases, G, u, v, by_order):
+    def __init__(self, plan, bases, G, u, v, by_order, prepared=False):
         h = _vp()
-        _check("bkx_bk3_mixed_graph_create", _bind().bkx_bk3_mixed_graph_create(
-            plan.handle, *plan._args(bases, G, u, v), int(by_order), ctypes.byref(h)))
+        name = "bkx_bk3_mixed_graph_create_prepared" if prepared else "bkx_bk3_mixed_graph_create"
+        _check(name, getattr(_bind(), name)(plan.handle, *plan._args(bases, G, u, v), int(by_order),
+                                            ctypes.byref(h)))
         self.handle = h
         self._keep = (plan, bases, G, u, v)
 

@@ -120,3 +120,21 @@ def test_sizes_and_errors():
     assert L.bkx_bk3_apply_prepared_f64(9, 1, None, None, None, None, None) == 1
     assert L.bkx_bk3_apply_prepared_f64(3, 1, None, None, None, None, None) == 2
     assert L.bkx_bk3_apply_prepared_f32w(3, 0, None, None, None, None, None) == 0
+
+
+def test_mixed_prepare_matches_per_order_forms():
+    from omp_bk_amd import mixed
+    L = _lib.lib()
+    bases = mixed.packed_bases()
+    P = mixed.prepared_bases(bases).numpy()
+    assert P.size == sum(L.bkx_bk3_basis_size(p) for p in range(1, 9))
+    off, boff = 0, 0
+    for p in range(1, 9):
+        nm, nq = p + 1, p + 2
+        st, form = _prepare(p, bases[boff:boff + nm * nq], bases[boff + nm * nq:boff + nm * nq + nq * nq])
+        assert st == 0
+        assert np.array_equal(P[off:off + form.size], form)
+        off += form.size
+        boff += nm * nq + nq * nq
+    with pytest.raises(ValueError):
+        mixed.prepared_bases(np.cos(np.arange(bases.size, dtype=np.float64)))

@@ -128,3 +128,36 @@ def test_full_batch_sampled():
     rng = np.random.default_rng(0)
     sample = np.concatenate([rng.choice(np.nonzero(order == p)[0], 40, replace=False) for p in range(1, 9)])
     _oracle_check(order, plan, bases, G, u, v1.cpu().numpy(), elems=sample)
+
+
+def test_prepared_bases_every_launch_form():
+    """The bases of every order in even-odd form: the single launch, the
+    persistent launch, the per-order launches and their graphs, against the
+    oracle per order and the direct launches (1e-14: the even-odd sums round
+    differently)."""
+    order, plan, bases, G, u = _batch(3000, seed=13)
+    b, g, uu = _t(bases), _t(G), _t(u)
+    P = mixed.prepared_bases(b)
+    direct = torch.full_like(uu, float("nan"))
+    plan.apply(b, g, uu, direct)
+    torch.cuda.synchronize()
+    big = direct.abs().max().item()
+    persist = mixed.MixedPlan(order, rounds_by_order=[2] * 9, grid=64)
+    for name, fn in (("single", lambda v: plan.apply(P, g, uu, v, prepared=True)),
+                     ("by_order", lambda v: plan.apply_by_order(P, g, uu, v, prepared=True)),
+                     ("persistent", lambda v: persist.apply(P, g, uu, v, prepared=True))):
+        print("mixed prepared", name, flush=True)
+        v = torch.full_like(uu, float("nan"))
+        fn(v)
+        torch.cuda.synchronize()
+        assert (v - direct).abs().max().item() <= 1e-14 * big, name
+        if name == "single":
+            _oracle_check(order, plan, bases, G, u, v.cpu().numpy())
+    for by_order in (False, True):
+        v = torch.full_like(uu, float("nan"))
+        graph = plan.graph(P, g, uu, v, by_order=by_order, prepared=True)
+        graph.launch()
+        torch.cuda.synchronize()
+        assert (v - direct).abs().max().item() <= 1e-14 * big, by_order
+        graph.close()
+    persist.close()
