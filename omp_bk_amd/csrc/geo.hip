@@ -15,7 +15,8 @@
 // BK3 / BK1 (Gauss points, q = p + 2): the reference gradient of each
 // coordinate component, computed by the phases of the transposing BK3
 // mapping (bk3_phases.hpp: interpolate with B, differentiate with the Gauss
-// collocation derivative D), one element per workgroup, J kept in the LDS;
+// collocation derivative D), geo_epb(p) elements per workgroup, J kept in
+// registers;
 // G is written factor-major, G(e, f, p, q, r), JxW as JxW(e, p, q, r).
 // BK5 (GLL nodes, collocated): J by the collocated derivative D(i, n) along
 // each direction from the coordinates staged in the LDS; G is written
@@ -61,29 +62,47 @@ __device__ __forceinline__ void factors_of(const double (&J)[3][3], double w, do
     }
 }
 
-template <int NM, int NQ>
+template <int NM, int NQ, int EPB = 1>
 struct GeoShape {
     static constexpr int nq2 = NQ * NQ;
     static constexpr int nq3 = nq2 * NQ;
-    static constexpr int threads = ((nq2 + kWave - 1) / kWave) * kWave;
+    static constexpr int threads = ((EPB * nq2 + kWave - 1) / kWave) * kWave;
 };
 
-// BK3 / BK1 points.  One element per workgroup; lane l < q^2 is the (q, r)
-// lane of the Gauss-point plane, holding column p.
-template <int NM, int NQ>
-__global__ __launch_bounds__((GeoShape<NM, NQ>::threads)) void geo_gauss_kernel(
+// Elements per workgroup of the Gauss-point kernel, the fastest measured on
+// MI355X at about 33 M DoF (ms for 1, 2, 3, 4, 8 and 16 elements, capped at
+// floor(256 / q^2)):
+//   p = 1: 4.94 3.26 2.69 2.38 2.28 2.21   p = 5: 0.80 0.84 0.84 0.85 0.80 0.80
+//   p = 2: 1.90 1.27 1.06 0.96 0.95 1.01   p = 6: 0.57 0.61 0.68 0.63 0.62 0.63
+//   p = 3: 1.44 1.13 1.27 1.16 1.12 1.15   p = 7: 1.04 1.07 0.92 0.94 0.93 0.91
+//   p = 4: 1.11 1.13 0.97 1.05 0.92 0.94   p = 8: 0.71 0.71 0.71 0.73 0.71 0.72
+// At p = 5, 6 and 8 the kernel's registers (J of three components for a
+// column of points) already limit the CU to a few elements.
+constexpr int geo_epb(int p)
+{
+    constexpr int epb[9] = {1, 16, 4, 8, 7, 1, 1, 3, 1};
+    return epb[p];
+}
+
+// BK3 / BK1 points.  EPB elements per workgroup; lane l < q^2 of slot s is
+// the (q, r) lane of element s's Gauss-point plane, holding column p.
+template <int NM, int NQ, int EPB>
+__global__ __launch_bounds__((GeoShape<NM, NQ, EPB>::threads)) void geo_gauss_kernel(
     size_t nelmt, const double* __restrict__ Bm, const double* __restrict__ Dm, const double* __restrict__ wq,
     const double* __restrict__ coords, double* __restrict__ G, double* __restrict__ JxW)
 {
     using E = bk3p::Elem<NM, NQ>;
-    using S = GeoShape<NM, NQ>;
-    __shared__ double s_box[3 * E::box];
+    using S = GeoShape<NM, NQ, EPB>;
+    __shared__ double s_box[EPB][3 * E::box];
 
-    const size_t e = blockIdx.x;
-    const int l = threadIdx.x;
-    const bool in_elem = l < S::nq2;
+    const int slot = threadIdx.x / S::nq2;
+    const int l = threadIdx.x - slot * S::nq2;
+    const bool in_elem = slot < EPB;
+    const size_t e = size_t(blockIdx.x) * EPB + slot;
+    const bool active = in_elem && e < nelmt;
+    const size_t ee = active ? e : 0;
     E el;
-    el.init(s_box, in_elem ? l : 0, in_elem);
+    el.init(s_box[in_elem ? slot : 0], l, in_elem);
     const bool modal = in_elem && l < E::nm2;
     // J(c, d) at the lane's column of points (p, lane), p < NQ, in registers:
     // the LDS then holds only the gradient's three boxes, so that several
@@ -92,7 +111,7 @@ __global__ __launch_bounds__((GeoShape<NM, NQ>::threads)) void geo_gauss_kernel(
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         double x[NM];
-        const double* xc = coords + (e * 3 + c) * E::nm3;
+        const double* xc = coords + (ee * 3 + c) * E::nm3;
         const double x0 = xc[0];
         const double* xe = xc + (modal ? l : 0);
 #pragma unroll
@@ -111,7 +130,7 @@ __global__ __launch_bounds__((GeoShape<NM, NQ>::threads)) void geo_gauss_kernel(
         // the next gradient writes box A, then box B only after a barrier,
         // so this lane's reads of B and C above come first
     }
-    if (!in_elem) {
+    if (!active) {
         return;
     }
     const double wqr = wq[el.lq] * wq[el.lq2];
@@ -142,7 +161,7 @@ __global__ __launch_bounds__((GeoShape<NM, NQ>::threads)) void geo_gauss_kernel(
 // BK5 points (GLL nodes, collocated).  One element per workgroup; lane
 // l = (j, k) < n^2 loops over i.
 template <int N>
-__global__ __launch_bounds__((GeoShape<N, N>::threads)) void geo_gll_kernel(
+__global__ __launch_bounds__((GeoShape<N, N, 1>::threads)) void geo_gll_kernel(
     size_t nelmt, const double* __restrict__ D, const double* __restrict__ wq, const double* __restrict__ coords,
     double* __restrict__ G)
 {
@@ -196,12 +215,14 @@ int geo_bk3_f64(int p, size_t nelmt, const double* B, const double* D, const dou
     return with_order<1, 8>(p, [&](auto P) {
         constexpr int NM = decltype(P)::value + 1;
         constexpr int NQ = NM + 1;
-        const unsigned grid = grid_for(nelmt, 1);
+        constexpr int EPB = geo_epb(NM - 1);
+        static_assert(EPB * NQ * NQ <= 1024, "one workgroup");
+        const unsigned grid = grid_for(nelmt, EPB);
         if (grid == 0) {
             return int(BKX_ERROR_SIZE);
         }
-        hipLaunchKernelGGL((geo_gauss_kernel<NM, NQ>), dim3(grid), dim3(GeoShape<NM, NQ>::threads), 0, s, nelmt, B, D,
-                           wq, coords, G, JxW);
+        hipLaunchKernelGGL((geo_gauss_kernel<NM, NQ, EPB>), dim3(grid), dim3(GeoShape<NM, NQ, EPB>::threads), 0, s,
+                           nelmt, B, D, wq, coords, G, JxW);
         return launch_status();
     });
 }
@@ -215,7 +236,7 @@ int geo_bk5_f64(int p, size_t nelmt, const double* D, const double* wq, const do
         if (grid == 0) {
             return int(BKX_ERROR_SIZE);
         }
-        hipLaunchKernelGGL((geo_gll_kernel<N>), dim3(grid), dim3(GeoShape<N, N>::threads), 0, s, nelmt, D, wq, coords,
+        hipLaunchKernelGGL((geo_gll_kernel<N>), dim3(grid), dim3(GeoShape<N, N, 1>::threads), 0, s, nelmt, D, wq, coords,
                            G);
         return launch_status();
     });

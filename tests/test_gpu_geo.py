@@ -54,6 +54,20 @@ def test_gauss_points_against_reference(p):
     assert _rel(J.cpu().numpy(), geo_ref.factors(1, p, c)) <= 1e-13
 
 
+@pytest.mark.parametrize("p,E", [(1, 33), (2, 9), (3, 17), (4, 15), (7, 8)])
+def test_gauss_points_partial_workgroups(p, E):
+    """Several elements per workgroup (16, 4, 8, 7 and 3 at these orders),
+    the last workgroup partly filled: every element's factors as computed
+    alone (the first element on its own), and the reference."""
+    c = _coords(p, E, seed=20 + p)
+    B, D = (_t(a) for a in geo.bk13_bases(p))
+    w = _t(geo.gauss(p + 2)[1])
+    G, J = ops.geo_bk3(p, B, D, w, _t(c))
+    assert _rel(G.cpu().numpy(), geo_ref.factors(3, p, c)) <= 1e-13
+    G1, J1 = ops.geo_bk3(p, B, D, w, _t(c[-1:]))
+    assert torch.equal(G1[0], G[-1]) and torch.equal(J1[0], J[-1])
+
+
 @pytest.mark.parametrize("p", range(1, 8))
 def test_gll_points_against_reference(p):
     E = 7
