@@ -19,8 +19,8 @@
 // registers;
 // G is written factor-major, G(e, f, p, q, r), JxW as JxW(e, p, q, r).
 // BK5 (GLL nodes, collocated): J by the collocated derivative D(i, n) along
-// each direction from the coordinates staged in the LDS; G is written
-// interleaved, G(e, i, j, f, k).
+// each direction from the coordinates staged in the LDS, as many elements
+// per wavefront as fit; G is written interleaved, G(e, i, j, f, k).
 //
 // The coordinates are differentiated relative to the element's first node,
 // x - x(0, 0, 0): the derivative matrices annihilate constants, and this
@@ -158,35 +158,78 @@ __global__ __launch_bounds__((GeoShape<NM, NQ, EPB>::threads)) void geo_gauss_ke
     }
 }
 
-// BK5 points (GLL nodes, collocated).  One element per workgroup; lane
-// l = (j, k) < n^2 loops over i.
+// BK5 points (GLL nodes, collocated).  gll_epb(n) elements per workgroup
+// of one wavefront (n^2 lanes each, so that the low orders fill it); lane
+// l = (j, k) < n^2 of slot s loops over i.
+constexpr int gll_epb(int n) { return kWave / (n * n) > 0 ? kWave / (n * n) : 1; }
+
 template <int N>
-__global__ __launch_bounds__((GeoShape<N, N, 1>::threads)) void geo_gll_kernel(
+__global__ __launch_bounds__((GeoShape<N, N, gll_epb(N)>::threads)) void geo_gll_kernel(
     size_t nelmt, const double* __restrict__ D, const double* __restrict__ wq, const double* __restrict__ coords,
     double* __restrict__ G)
 {
     constexpr int n2 = N * N;
     constexpr int n3 = n2 * N;
-    __shared__ double sx[3 * n3];
-    const size_t e = blockIdx.x;
-    const int l = threadIdx.x;
-    for (int t = l; t < 3 * n3; t += blockDim.x) {
-        const double* xc = coords + e * 3 * n3 + (t / n3) * n3;
-        sx[t] = xc[t % n3] - xc[0];
+    constexpr int EPB = gll_epb(N);
+    constexpr int T = GeoShape<N, N, EPB>::threads;
+    constexpr int per = (EPB * 3 * n3 + T - 1) / T;   // coordinates staged per lane
+    __shared__ double sx[EPB * 3 * n3];
+    const size_t e0 = size_t(blockIdx.x) * EPB;
+    const int l0 = threadIdx.x;
+    // the workgroup's elements are consecutive: all of a lane's loads in
+    // flight at once (clamped to the last element), then relative to each
+    // element's node 0
+    const double* xe = coords + e0 * 3 * n3;
+    double xv[per];
+    if constexpr (EPB == 1) {
+#pragma unroll
+        for (int r = 0; r < per; ++r) {
+            const int t = l0 + r * T;
+            xv[r] = xe[t < 3 * n3 ? t : 0];
+        }
+        const double x0[3] = {xe[0], xe[n3], xe[2 * n3]};
+#pragma unroll
+        for (int r = 0; r < per; ++r) {
+            const int t = l0 + r * T;
+            if (t < 3 * n3) {
+                sx[t] = xv[r] - x0[t / n3];
+            }
+        }
+    } else {
+        const size_t nv = (nelmt - e0 < size_t(EPB) ? nelmt - e0 : size_t(EPB)) * 3 * n3;
+        double xz[per];
+#pragma unroll
+        for (int r = 0; r < per; ++r) {
+            const int t = l0 + r * T;
+            const int tc = size_t(t) < nv ? t : 0;
+            xv[r] = xe[tc];
+            xz[r] = xe[(tc / (3 * n3)) * 3 * n3 + ((tc / n3) % 3) * n3];
+        }
+#pragma unroll
+        for (int r = 0; r < per; ++r) {
+            const int t = l0 + r * T;
+            if (t < EPB * 3 * n3) {
+                sx[t] = xv[r] - xz[r];
+            }
+        }
     }
     __syncthreads();
-    if (l >= n2) {
+    const int slot = l0 / n2;
+    const int l = l0 - slot * n2;
+    const size_t e = e0 + slot;
+    if (slot >= EPB || e >= nelmt) {
         return;
     }
     const int j = l / N;
     const int k = l - j * N;
     const auto Dr = launder<1>(D);
+    const double* sxe = sx + slot * 3 * n3;
 #pragma unroll 1
     for (int i = 0; i < N; ++i) {
         double J[3][3];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const double* xc = sx + c * n3;
+            const double* xc = sxe + c * n3;
             double d0 = 0.0, d1 = 0.0, d2 = 0.0;
 #pragma unroll
             for (int m = 0; m < N; ++m) {
@@ -202,7 +245,7 @@ __global__ __launch_bounds__((GeoShape<N, N, 1>::threads)) void geo_gll_kernel(
         factors_of(J, wq[i] * wq[j] * wq[k], g, wdet);
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
-            G[e * 6 * n3 + ((i * N + j) * 6 + f) * N + k] = g[f];
+            st_stream(G + e * 6 * n3 + ((i * N + j) * 6 + f) * N + k, g[f]);
         }
     }
 }
@@ -232,12 +275,13 @@ int geo_bk5_f64(int p, size_t nelmt, const double* D, const double* wq, const do
 {
     return with_order<1, 7>(p, [&](auto P) {
         constexpr int N = decltype(P)::value + 1;
-        const unsigned grid = grid_for(nelmt, 1);
+        constexpr int EPB = gll_epb(N);
+        const unsigned grid = grid_for(nelmt, EPB);
         if (grid == 0) {
             return int(BKX_ERROR_SIZE);
         }
-        hipLaunchKernelGGL((geo_gll_kernel<N>), dim3(grid), dim3(GeoShape<N, N, 1>::threads), 0, s, nelmt, D, wq, coords,
-                           G);
+        hipLaunchKernelGGL((geo_gll_kernel<N>), dim3(grid), dim3(GeoShape<N, N, EPB>::threads), 0, s, nelmt, D, wq,
+                           coords, G);
         return launch_status();
     });
 }

@@ -68,6 +68,20 @@ def test_gauss_points_partial_workgroups(p, E):
     assert torch.equal(G1[0], G[-1]) and torch.equal(J1[0], J[-1])
 
 
+@pytest.mark.parametrize("p,E", [(1, 33), (2, 9), (3, 17), (4, 5)])
+def test_gll_points_partial_workgroups(p, E):
+    """Several elements per wavefront at the low orders (16, 7, 4 and 2),
+    the last workgroup partly filled: the reference, and the last element
+    computed alone."""
+    c = _coords(p, E, seed=30 + p)
+    D = _t(geo.bk5_basis(p))
+    w = _t(geo.gll(p + 1)[1])
+    G = ops.geo_bk5(p, D, w, _t(c))
+    assert _rel(G.cpu().numpy(), geo_ref.factors(5, p, c)) <= 1e-13
+    G1 = ops.geo_bk5(p, D, w, _t(c[-1:]))
+    assert torch.equal(G1[0], G[-1])
+
+
 @pytest.mark.parametrize("p", range(1, 8))
 def test_gll_points_against_reference(p):
     E = 7
