@@ -428,13 +428,13 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
         if (rounds > 1) {
             pl->persistent = true;
         }
-        const int per_tile = epb * rounds;
+        const int per_tile = int(std::min<int64_t>(int64_t(epb) * rounds, INT32_MAX));
         pl->div_p[p] = div;
         const int first = int(elems.size());
         pl->elem_begin[p] = first;
         elems.insert(elems.end(), by_p[p].begin(), by_p[p].end());
         const int cnt = int(by_p[p].size());
-        for (int t = 0; t < cnt; t += per_tile) {
+        for (int t = 0; t < cnt; t += std::min(per_tile, cnt - t)) {
             const int c = std::min(per_tile, cnt - t);
             grouped.push_back(bkx::MixedTile{p, div, first + t, c});
             const int n = bkx::n_of(p);
