@@ -257,18 +257,38 @@ def _median_ms(fn, steps=10, warmup=2):
     return statistics.median(out)
 
 
+def _batch_ms(fn, steps=10, reps=10, warmup=2):
+    """Median over steps of the mean time of reps back-to-back calls: the
+    steady state of a launch inside a sequence of launches, which for short
+    launches differs from a launch timed alone on an idle GPU."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    s = torch.cuda.current_stream()
+    out = []
+    for _ in range(steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        for _ in range(reps):
+            fn()
+        b.record(s)
+        b.synchronize()
+        out.append(a.elapsed_time(b) / reps)
+    return statistics.median(out)
+
+
 def tune(order, bases, G, u, v, steps=10):
     """For the per-order launches: per order, the elements-per-workgroup
     divisor (1 or 2) whose launch of that order's elements (by the kernel
-    compiled for that order) is fastest; returns (div_by_order,
-    {p: {div: ms}})."""
+    compiled for that order) is fastest, timed back to back as in the
+    per-order sequence; returns (div_by_order, {p: {div: ms}})."""
     plans = {d: MixedPlan(order, div_by_order=[d] * 9) for d in (1, 2)}
     best = [1] * 9
     times = {}
     for p in range(1, 9):
         if not np.any(order == p):
             continue
-        times[p] = {d: _median_ms(lambda: pl.apply_order(p, bases, G, u, v), steps) for d, pl in plans.items()}
+        times[p] = {d: _batch_ms(lambda: pl.apply_order(p, bases, G, u, v), steps) for d, pl in plans.items()}
         best[p] = min(times[p], key=times[p].get)
     for pl in plans.values():
         pl.close()
