@@ -16,6 +16,7 @@ import numpy as np
 import torch
 
 from . import _lib
+from .ops import _on, _stream
 
 
 class Restriction:
@@ -52,9 +53,9 @@ class Restriction:
         if out is None:
             out = torch.empty(self.nelmt * self.elemsize, dtype=torch.float64, device=uL.device)
         self._check("out", out, self.nelmt * self.elemsize)
-        with torch.cuda.device(uL.device):
+        with _on(uL):
             _lib.call("bkx_restrict_gather_f64", self._plan, uL.data_ptr(), out.data_ptr(),
-                      torch.cuda.current_stream(uL.device).cuda_stream)
+                      _stream(uL))
         return out
 
     def scatter(self, vE, out=None):
@@ -63,9 +64,9 @@ class Restriction:
         if out is None:
             out = torch.empty(self.lsize, dtype=torch.float64, device=vE.device)
         self._check("out", out, self.lsize)
-        with torch.cuda.device(vE.device):
+        with _on(vE):
             _lib.call("bkx_restrict_scatter_f64", self._plan, vE.data_ptr(), out.data_ptr(),
-                      torch.cuda.current_stream(vE.device).cuda_stream)
+                      _stream(vE))
         return out
 
     def bk5(self, p, D, G, uL, out=None):
@@ -82,7 +83,7 @@ class Restriction:
         if out is None:
             out = torch.empty(self.nelmt * self.elemsize, dtype=torch.float64, device=uL.device)
         self._check("out", out, self.nelmt * self.elemsize)
-        with torch.cuda.device(uL.device):
+        with _on(uL):
             _lib.call("bkx_bk5_apply_restricted_f64", self._plan, p, D.data_ptr(), G.data_ptr(), uL.data_ptr(),
-                      out.data_ptr(), torch.cuda.current_stream(uL.device).cuda_stream)
+                      out.data_ptr(), _stream(uL))
         return out
