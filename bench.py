@@ -338,7 +338,9 @@ def mixed_batch(steps, warmup, nelmt=200000):
     the single launch with divisors tuned per order (one workgroup per
     tile), the per-order launches, and the single launch as a persistent
     loop over the tiles (one workgroup per CU slot), each replayed from a
-    hipGraph; the first two also with the bases in even-odd form (_eo)."""
+    hipGraph; the first two also with the bases in even-odd form (_eo); and
+    the same elements in the grouped layout (_grouped: each order's elements
+    contiguous, each order's range 256-byte aligned)."""
     import numpy as np
     import torch
     from omp_bk_amd import mixed
@@ -355,20 +357,33 @@ def mixed_batch(steps, warmup, nelmt=200000):
     nbytes = 8 * (2 * plan0.ndof + plan0.ngeo)
     out = {"nelmt": nelmt, "counts_by_p": np.bincount(order, minlength=9)[1:].tolist(), "bytes": nbytes}
     P = mixed.prepared_bases(bases)
-    for name, by_order, div, grid, prep in (("single_launch_graph", False, div1, 0, False),
-                                            ("per_order_graph", True, divp, 0, False),
-                                            ("persistent_graph", False, div1, plan0.grid, False),
-                                            ("single_launch_graph_eo", False, div1, 0, True),
-                                            ("per_order_graph_eo", True, divp, 0, True)):
-        plan = mixed.MixedPlan(order, div_by_order=div, grid=grid)
-        g = plan.graph(P if prep else bases, G, u, v, by_order=by_order, prepared=prep)
+    # the grouped layout: the same elements, each order's contiguous and
+    # 256-byte aligned (their own geometry and inputs, placed by its offsets)
+    gplan0 = mixed.MixedPlan(order, grouped=True)
+    Gg = mixed.affine_factors(order, gplan0.goff, gplan0.ngeo, device=dev)
+    ug = torch.rand(gplan0.ndof, dtype=torch.float64, device=dev) * 2 - 1
+    vg = torch.empty_like(ug)
+    for name, by_order, div, grid, prep, grouped in (
+            ("single_launch_graph", False, div1, 0, False, False),
+            ("per_order_graph", True, divp, 0, False, False),
+            ("persistent_graph", False, div1, plan0.grid, False, False),
+            ("single_launch_graph_eo", False, div1, 0, True, False),
+            ("per_order_graph_eo", True, divp, 0, True, False),
+            ("single_launch_graph_grouped", False, div1, 0, False, True),
+            ("per_order_graph_grouped", True, divp, 0, False, True),
+            ("single_launch_graph_grouped_eo", False, div1, 0, True, True),
+            ("per_order_graph_grouped_eo", True, divp, 0, True, True)):
+        plan = mixed.MixedPlan(order, div_by_order=div, grid=grid, grouped=grouped)
+        arrays = (Gg, ug, vg) if grouped else (G, u, v)
+        g = plan.graph(P if prep else bases, *arrays, by_order=by_order, prepared=prep)
         ms, _ = time_steps(g.launch, steps, warmup)
         g.close()
-        out[name] = {"ms": round(ms, 4), "GDoF/s": round(plan.ndof / (ms * 1e-3) * 1e-9, 3),
+        out[name] = {"ms": round(ms, 4), "GDoF/s": round(plan.dofs / (ms * 1e-3) * 1e-9, 3),
                      "GB/s": round(nbytes / (ms * 1e-3) * 1e-9, 1), "div_by_p": div[1:]}
         if not by_order:
             out[name]["ntiles"] = plan.ntiles
         plan.close()
+    gplan0.close()
     plan0.close()
     return out
 

@@ -284,8 +284,9 @@ int bkx_bk5_apply_restricted_f64(const bkx_restrict* plan, int p, const double* 
 /* A batch holds nelmt elements in the caller's order, element e of order   */
 /* p_e in 1..8 (n = p + 1, q = p + 2): its n^3 values of u / v start at     */
 /* uoff[e] = sum_{e' < e} n_e'^3 and its 6 q^3 factors of G (factor-major,  */
-/* as bkx_bk3_apply_f64) at goff[e] = sum_{e' < e} 6 q_e'^3.  `bases` packs */
-/* B_p (n x q) then D_p (q x q) for p = 1, 2, ..., 8 in turn.               */
+/* as bkx_bk3_apply_f64) at goff[e] = sum_{e' < e} 6 q_e'^3 (or, for a      */
+/* grouped plan, grouped by order).  `bases` packs B_p (n x q) then D_p     */
+/* (q x q) for p = 1, 2, ..., 8 in turn.                                    */
 /* ---------------------------------------------------------------------- */
 
 typedef struct bkx_mixed_plan bkx_mixed_plan;
@@ -298,8 +299,15 @@ typedef struct bkx_graph bkx_graph;
  * grid <= 0 picks the number of resident workgroups of the device. */
 int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
                           const int32_t* rounds_by_order, int grid, bkx_mixed_plan** plan);
+/* The same plan for the grouped layout: each order's elements contiguous
+ * (ascending element index, orders ascending), each order's range of u/v
+ * and of G starting on a 256-byte boundary; the offsets (with the padding
+ * between the ranges) are those of bkx_mixed_plan_offsets. */
+int bkx_mixed_plan_create_grouped(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
+                                  const int32_t* rounds_by_order, int grid, bkx_mixed_plan** plan);
 int bkx_mixed_plan_destroy(bkx_mixed_plan* plan);
-/* Sizes of u/v (ndof) and G (ngeo) in doubles, the tile count and grid. */
+/* Sizes of u/v (ndof) and G (ngeo) in doubles, padding included, the tile
+ * count and grid. */
 int bkx_mixed_plan_info(const bkx_mixed_plan* plan, size_t* ndof, size_t* ngeo, int* ntiles, int* grid);
 /* The element offsets uoff, goff (host arrays of nelmt). */
 int bkx_mixed_plan_offsets(const bkx_mixed_plan* plan, int64_t* uoff, int64_t* goff);

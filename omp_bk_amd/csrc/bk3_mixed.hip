@@ -380,10 +380,40 @@ int graph_create(const bkx_mixed_plan* plan, const double* bases, const double* 
 
 }  // namespace
 
+namespace {
+
+// Offsets of the grouped layout: each order's elements contiguous, in
+// ascending element index, orders ascending, each order's range of u/v and
+// of G starting on a 256-byte boundary.
+constexpr int64_t kGroupAlign = 32;   // doubles
+
+int64_t align_up(int64_t x) { return (x + kGroupAlign - 1) / kGroupAlign * kGroupAlign; }
+
+int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order, const int32_t* rounds_by_order,
+                int grid, bool group_layout, bkx_mixed_plan** plan);
+
+}  // namespace
+
 extern "C" {
 
 int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
                           const int32_t* rounds_by_order, int grid, bkx_mixed_plan** plan)
+{
+    return plan_create(order, nelmt, div_by_order, rounds_by_order, grid, false, plan);
+}
+
+int bkx_mixed_plan_create_grouped(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
+                                  const int32_t* rounds_by_order, int grid, bkx_mixed_plan** plan)
+{
+    return plan_create(order, nelmt, div_by_order, rounds_by_order, grid, true, plan);
+}
+
+}  // extern "C"
+
+namespace {
+
+int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order, const int32_t* rounds_by_order,
+                int grid, bool group_layout, bkx_mixed_plan** plan)
 {
     if (!plan || (nelmt > 0 && !order)) {
         return BKX_ERROR_NULL;
@@ -412,6 +442,20 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
         uo += int64_t(bkx::n_of(p)) * bkx::n_of(p) * bkx::n_of(p);
         go += int64_t(6) * bkx::q_of(p) * bkx::q_of(p) * bkx::q_of(p);
         by_p[p].push_back(int32_t(e));
+    }
+    if (group_layout) {
+        uo = 0;
+        go = 0;
+        for (int p = 1; p <= 8; ++p) {
+            uo = align_up(uo);
+            go = align_up(go);
+            for (const int32_t e : by_p[p]) {
+                pl->uoff_h[size_t(e)] = uo;
+                pl->goff_h[size_t(e)] = go;
+                uo += int64_t(bkx::n_of(p)) * bkx::n_of(p) * bkx::n_of(p);
+                go += int64_t(6) * bkx::q_of(p) * bkx::q_of(p) * bkx::q_of(p);
+            }
+        }
     }
     pl->ndof = size_t(uo);
     pl->ngeo = size_t(go);
@@ -511,6 +555,10 @@ int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div
     *plan = pl;
     return BKX_SUCCESS;
 }
+
+}  // namespace
+
+extern "C" {
 
 int bkx_mixed_plan_destroy(bkx_mixed_plan* plan)
 {

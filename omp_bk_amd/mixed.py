@@ -33,6 +33,7 @@ def _bind():
     if getattr(L, "_bkx_mixed_bound", False):
         return L
     L.bkx_mixed_plan_create.argtypes = [_i32p, ctypes.c_size_t, _i32p, _i32p, ctypes.c_int, ctypes.POINTER(_vp)]
+    L.bkx_mixed_plan_create_grouped.argtypes = L.bkx_mixed_plan_create.argtypes
     L.bkx_mixed_plan_destroy.argtypes = [_vp]
     L.bkx_mixed_plan_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
@@ -49,7 +50,7 @@ def _bind():
     L.bkx_bk3_mixed_graph_create_prepared.argtypes = [_vp] + [_vp] * 4 + [ctypes.c_int, ctypes.POINTER(_vp)]
     L.bkx_graph_launch.argtypes = [_vp, _vp]
     L.bkx_graph_destroy.argtypes = [_vp]
-    for name in ("bkx_mixed_plan_create", "bkx_mixed_plan_destroy", "bkx_mixed_plan_info", "bkx_mixed_plan_offsets",
+    for name in ("bkx_mixed_plan_create", "bkx_mixed_plan_create_grouped", "bkx_mixed_plan_destroy", "bkx_mixed_plan_info", "bkx_mixed_plan_offsets",
                  "bkx_bk3_mixed_apply_f64", "bkx_bk3_mixed_apply_by_order_f64", "bkx_bk3_mixed_apply_order_f64",
                  "bkx_bk3_mixed_graph_create", "bkx_graph_launch", "bkx_graph_destroy", "bkx_bk3_mixed_prepare",
                  "bkx_bk3_mixed_apply_prepared_f64", "bkx_bk3_mixed_apply_by_order_prepared_f64",
@@ -151,9 +152,13 @@ def affine_factors(order, goff, ngeo, amp=0.1, seed=7, device=None):
 # ---------------------------------------------------------------------------
 
 class MixedPlan:
-    """The device-side tile lists of one batch (bkx_mixed_plan_create)."""
+    """The device-side tile lists of one batch (bkx_mixed_plan_create).  With
+    grouped, the batch's arrays are in the grouped layout
+    (bkx_mixed_plan_create_grouped): each order's elements contiguous, each
+    order's range on a 256-byte boundary; uoff and goff give every element's
+    place either way."""
 
-    def __init__(self, order, div_by_order=None, rounds_by_order=None, grid=0):
+    def __init__(self, order, div_by_order=None, rounds_by_order=None, grid=0, grouped=False):
         L = _bind()
         self.order = np.ascontiguousarray(order, dtype=np.int32)
         self.nelmt = self.order.size
@@ -161,10 +166,13 @@ class MixedPlan:
         rnd = None if rounds_by_order is None else np.ascontiguousarray(rounds_by_order, dtype=np.int32)
         self._div, self._rnd = div, rnd
         h = _vp()
-        _check("bkx_mixed_plan_create", L.bkx_mixed_plan_create(
+        name = "bkx_mixed_plan_create_grouped" if grouped else "bkx_mixed_plan_create"
+        _check(name, getattr(L, name)(
             self.order.ctypes.data_as(_i32p), self.nelmt, None if div is None else div.ctypes.data_as(_i32p),
             None if rnd is None else rnd.ctypes.data_as(_i32p), int(grid), ctypes.byref(h)))
         self.handle = h
+        self.grouped = bool(grouped)
+        self.dofs = int(np.sum((self.order.astype(np.int64) + 1) ** 3))   # without padding
         nd, ng, nt, gr = ctypes.c_size_t(), ctypes.c_size_t(), ctypes.c_int(), ctypes.c_int()
         _check("bkx_mixed_plan_info", L.bkx_mixed_plan_info(h, ctypes.byref(nd), ctypes.byref(ng), ctypes.byref(nt),
                                                             ctypes.byref(gr)))
@@ -187,9 +195,11 @@ class MixedPlan:
 
     def _args(self, bases, G, u, v):
         # bases: the packed bases, or their even-odd forms (prepared_bases)
-        for t in (bases, G, u, v):
-            assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()
-        assert u.numel() == self.ndof and v.numel() == self.ndof and G.numel() == self.ngeo
+        for name, t in (("bases", bases), ("G", G), ("u", u), ("v", v)):
+            if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
+                raise TypeError("%s must be a contiguous float64 tensor on a HIP device" % name)
+        if u.numel() != self.ndof or v.numel() != self.ndof or G.numel() != self.ngeo:
+            raise ValueError("u and v must hold %d values and G %d" % (self.ndof, self.ngeo))
         return bases.data_ptr(), G.data_ptr(), u.data_ptr(), v.data_ptr()
 
     def apply(self, bases, G, u, v, prepared=False):

@@ -161,3 +161,55 @@ def test_prepared_bases_every_launch_form():
         assert (v - direct).abs().max().item() <= 1e-14 * big, by_order
         graph.close()
     persist.close()
+
+
+def test_grouped_layout_same_bits():
+    """A plan for the grouped layout (each order's elements contiguous, each
+    order's range on a 256-byte boundary): the same elements' data placed
+    by its offsets give the bits of the caller-order plan, element by
+    element, in every launch form."""
+    order, plan, bases, G, u = _batch(3000, seed=21)
+    gp = mixed.MixedPlan(order, grouped=True)
+    assert gp.dofs == plan.dofs == plan.ndof and gp.ndof >= plan.ndof and gp.ngeo >= plan.ngeo
+    for p in range(1, 9):
+        idx = np.nonzero(order == p)[0]
+        n3, g3 = (p + 1) ** 3, 6 * (p + 2) ** 3
+        assert gp.uoff[idx[0]] % 32 == 0 and gp.goff[idx[0]] % 32 == 0
+        assert np.array_equal(np.diff(gp.uoff[idx]), np.full(idx.size - 1, n3))
+        assert np.array_equal(np.diff(gp.goff[idx]), np.full(idx.size - 1, g3))
+    ug = np.zeros(gp.ndof)
+    Gg = np.zeros(gp.ngeo)
+    for e in range(order.size):
+        n3, g3 = (order[e] + 1) ** 3, 6 * (order[e] + 2) ** 3
+        ug[gp.uoff[e]:gp.uoff[e] + n3] = u[plan.uoff[e]:plan.uoff[e] + n3]
+        Gg[gp.goff[e]:gp.goff[e] + g3] = G[plan.goff[e]:plan.goff[e] + g3]
+    b, g, uu = _t(bases), _t(G), _t(u)
+    bg, ugt = _t(Gg), _t(ug)
+    ref = torch.empty_like(uu)
+    plan.apply(b, g, uu, ref)
+    ref = ref.cpu().numpy()
+    P = mixed.prepared_bases(b)
+    refp = torch.empty_like(uu)
+    plan.apply(P, g, uu, refp, prepared=True)
+    refp = refp.cpu().numpy()
+    persist = mixed.MixedPlan(order, rounds_by_order=[2] * 9, grid=64, grouped=True)
+    runs = [(gp.apply, False), (gp.apply_by_order, False), (persist.apply, False), (gp.apply, True),
+            (gp.apply_by_order, True)]
+    for fn, prep in runs:
+        v = torch.full_like(ugt, float("nan"))
+        fn(P if prep else b, bg, ugt, v, prepared=prep)
+        v = v.cpu().numpy()
+        want = refp if prep else ref
+        for e in range(order.size):
+            n3 = (order[e] + 1) ** 3
+            assert np.array_equal(v[gp.uoff[e]:gp.uoff[e] + n3], want[plan.uoff[e]:plan.uoff[e] + n3]), (fn, prep, e)
+    v = torch.full_like(ugt, float("nan"))
+    graph = gp.graph(b, bg, ugt, v, by_order=True)
+    graph.launch()
+    torch.cuda.synchronize()
+    got = v.cpu().numpy()
+    assert all(np.array_equal(got[gp.uoff[e]:gp.uoff[e] + (order[e] + 1) ** 3],
+                              ref[plan.uoff[e]:plan.uoff[e] + (order[e] + 1) ** 3]) for e in range(order.size))
+    graph.close()
+    persist.close()
+    gp.close()
