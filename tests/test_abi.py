@@ -63,6 +63,15 @@ def test_error_paths_without_device():
     assert L.bkx_restrict_gather_f64(None, None, None, None) == 2
     assert L.bkx_restrict_scatter_f64(None, None, None, None) == 2
     assert L.bkx_bk5_apply_restricted_f64(None, 7, None, None, None, None, None) == 2
+    # mixed-order plans: checked on the host before any device array is made
+    from omp_bk_amd import mixed
+    M = mixed._bind()
+    bad = np.array([3, 9], dtype=np.int32)
+    pbad = bad.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    for create in (M.bkx_mixed_plan_create, M.bkx_mixed_plan_create_grouped):
+        assert create(pbad, 2, None, None, 0, None) == 2
+        assert create(None, 2, None, None, 0, ctypes.byref(h)) == 2
+        assert create(pbad, 2, None, None, 0, ctypes.byref(h)) == 1
 
 
 def test_python_errors():
