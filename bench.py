@@ -371,6 +371,7 @@ def mixed_batch(steps, warmup, nelmt=200000):
             ("per_order_graph_eo", True, divp, 0, True, False),
             ("single_launch_graph_grouped", False, div1, 0, False, True),
             ("per_order_graph_grouped", True, divp, 0, False, True),
+            ("persistent_graph_grouped", False, div1, plan0.grid, False, True),
             ("single_launch_graph_grouped_eo", False, div1, 0, True, True),
             ("per_order_graph_grouped_eo", True, divp, 0, True, True)):
         plan = mixed.MixedPlan(order, div_by_order=div, grid=grid, grouped=grouped)
