@@ -88,7 +88,7 @@ struct VariantTable {
         }
         //                            p:   0   1   2   3   4   5   6   7   8
         static const int bk5_f64[kOrders] = {0, 27, 18, 35, 34, 36, 38, 36, 0};
-        static const int bk5_f32w[kOrders] = {0, 10, 16, 35, 27, 39, 22, 41, 0};
+        static const int bk5_f32w[kOrders] = {0, 10, 16, 35, 27, 39, 22, 34, 0};
         static const int bk3_f64[kOrders] = {0, 16, 14, 16, 17, 16, 13, 16, 18};
         static const int bk1_f64[kOrders] = {0, 5, 1, 19, 18, 19, 18, 19, 18};
         static const int bk3_f32w[kOrders] = {0, 9, 6, 8, 8, 10, 4, 8, 8};
