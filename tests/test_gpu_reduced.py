@@ -2,8 +2,8 @@
 inputs (u and G rounded to FP32 are exact FP64 values, so the oracle is the
 exact operator of the stored data up to FP64 rounding).
 
-Bars: f32c (float-float arithmetic) <= 1e-12 max-relative, the north-star
-bound for the FP32-compensated form; f32w (FP64 arithmetic) <= 1e-13.
+Bars: f32c (float-float arithmetic) <= 1e-12 max-relative, the bound the
+FP32-compensated form is specified to; f32w (FP64 arithmetic) <= 1e-13.
 """
 import numpy as np
 import pytest
