@@ -33,7 +33,8 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk5_bf16s", "bk3", "copy"], default=None,
+    ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk5_bf16s", "bk3", "copy", "mixed"],
+                    default=None,
                     help="run one workload only (for profiling) and print its timing")
     return ap.parse_args()
 
@@ -523,11 +524,54 @@ def traffic(name):
         return None
 
 
+def mixed_layouts(steps, warmup, nelmt=200000):
+    """The 200,000-element batch as one launch per order (divisor 1), from a
+    hipGraph, in the caller-order and in the grouped layout; and the plain
+    BK3 operator on each order's element count alone (contiguous arrays):
+    the per-order kernels of the three can be compared in a kernel trace."""
+    import numpy as np
+    import torch
+    from omp_bk_amd import geometry as geo
+    from omp_bk_amd import mixed
+    from omp_bk_amd import ops
+    dev = "cuda:0"
+    order = mixed.zipf_orders(nelmt)
+    bases = torch.as_tensor(mixed.packed_bases(), device=dev)
+    out = {}
+    for name, grouped in (("per_order_graph", False), ("per_order_graph_grouped", True)):
+        plan = mixed.MixedPlan(order, grouped=grouped)
+        G = mixed.affine_factors(order, plan.goff, plan.ngeo, device=dev)
+        u = torch.rand(plan.ndof, dtype=torch.float64, device=dev) * 2 - 1
+        v = torch.empty_like(u)
+        g = plan.graph(bases, G, u, v, by_order=True)
+        ms, _ = time_steps(g.launch, steps, warmup)
+        g.close()
+        out[name] = round(ms, 4)
+        plan.close()
+        del G, u, v
+    counts = np.bincount(order, minlength=9)
+    alone = {}
+    for p in range(1, 9):
+        E, nm, nq = int(counts[p]), p + 1, p + 2
+        B, D = (torch.as_tensor(x, device=dev) for x in geo.bk13_bases(p))
+        G = torch.rand(E * 6 * nq ** 3, dtype=torch.float64, device=dev)
+        u = torch.rand(E * nm ** 3, dtype=torch.float64, device=dev)
+        v = torch.empty_like(u)
+        ms, _ = time_steps(lambda: ops.bk3(p, B, D, G, u, out=v), steps, warmup)
+        alone[p] = round(ms, 4)
+    out["bk3_alone_ms_by_p"] = alone
+    out["bk3_alone_sum_ms"] = round(sum(alone.values()), 4)
+    return out
+
+
 def run_only(a):
     """One workload, a few steps, for rocprofv3 runs."""
     from omp_bk_amd import _lib
     from omp_bk_amd import ops
     import torch
+    if a.only == "mixed":
+        print(json.dumps({"only": "mixed", **mixed_layouts(a.steps, a.warmup)}), flush=True)
+        return
     if a.only == "copy":
         n = (2 << 30) // 8
         x = torch.ones(n, dtype=torch.float64, device="cuda:0")
