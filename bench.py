@@ -282,12 +282,13 @@ def rate(w, ms):
             "bytes_per_dof": round(w["bytes"] / w["dofs"], 3)}
 
 
-def adjoint_config(steps, warmup, p=6, E=16384):
+def adjoint_config(steps, warmup, p=6, E=16384, forms=("f64", "f32w"), ms64=None):
     """The BK3 adjoint configuration: L = 1/2 |A(kappa) u - b|^2 with kappa
     log-normal (sigma = 0.5), u, b = U[-1, 1], p = 6, E = 16384; one step is
     the fused adjoint (dL/du, dL/dkappa and the per-element loss) plus the
     fixed-order sum of L.  FP64, and FP32 storage of u, b, kappa and G with
-    FP64 arithmetic (errors against FP64 on the same rounded inputs).  Bytes
+    FP64 arithmetic (f32w) or float-float compensated arithmetic (f32c);
+    errors against FP64 on the same rounded inputs.  Bytes
     per DoF count every input once and both outputs (G and kappa are read a
     second time from the L2 cache)."""
     import torch
@@ -304,23 +305,33 @@ def adjoint_config(steps, warmup, p=6, E=16384):
     dofs = E * n ** 3
     out = {}
     ref = None
-    for form, s in (("f64", 8), ("f32w", 4)):
-        cast = (lambda x: x.float()) if form == "f32w" else (lambda x: x)
+    for form in forms:
+        s = 8 if form == "f64" else 4
+        cast = (lambda x: x.float()) if form != "f64" else (lambda x: x)
         Gs, ks, us, bs = cast(G), cast(kappa), cast(u), cast(b)
-        if form == "f32w":
+        if form != "f64":
             ref = ops.bk3_adjoint(p, B, D, Gs.double(), ks.double(), us.double(), bs.double())
         res = {}
         fn = lambda: res.__setitem__("r", ops.bk3_adjoint(p, B, D, Gs, ks, us, bs, form=form))
         ms, _ = time_steps(fn, steps, warmup)
         nbytes = s * (2 * dofs + 7 * E * q ** 3) + 8 * (dofs + E * q ** 3)
         line = {"value": round(dofs / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
-                "bytes_per_dof": round(nbytes / dofs, 2), "achieved_GBs": round(nbytes / (ms * 1e-3) * 1e-9, 1)}
+                "bytes_per_dof": round(nbytes / dofs, 2), "achieved_GBs": round(nbytes / (ms * 1e-3) * 1e-9, 1),
+                "arithmetic": {"f64": "FP64", "f32w": "FP64 (FP32 storage)",
+                               "f32c": "float-float compensated (FP32 storage)"}[form]}
         if ref is not None:
             L, du, dk, _ = res["r"]
             rel = lambda x, r: float(((x - r).abs().max() / r.abs().max()).item())
             line["relerr_vs_f64_same_inputs"] = {"L": abs(L.item() - ref[0].item()) / ref[0].item(),
                                                  "dLdu": rel(du, ref[1]), "dLdkappa": rel(dk, ref[2])}
+        if form == "f64":
+            ms64 = ms
+        elif ms64 is not None:
+            line["speedup_vs_f64"] = round(ms64 / ms, 3)
         out["bk3_adjoint_p6_" + form] = line
+        if form == "f32c":
+            del Gs, ks, us, bs
+            continue
         # the same with the bases in even-odd form
         P = ops.bk3_basis(p, B, D)
         ms_eo, _ = time_steps(lambda: ops.bk3_adjoint_prepared(p, P, Gs, ks, us, bs, form=form), steps, warmup)
@@ -644,6 +655,8 @@ def main():
     mix = mixed_batch(a.steps, a.warmup)
 
     forms.update(reduced_forms(w5, a.steps, a.warmup, ms5, forms=("f32c",)))
+    lines.update(adjoint_config(a.steps, a.warmup, forms=("f32c",),
+                                ms64=lines["bk3_adjoint_p6_f64"]["ms_per_step"]))
 
     cpu = None if a.no_cpu else cpu_baseline(w5)
 

@@ -220,6 +220,23 @@ int bkx_bk3_vjp_f32w(int p, size_t nelmt, const double* basis, const double* dba
                      const float* kappa, const float* u, const float* w, double* grad_u, double* grad_kappa,
                      bkx_stream_t stream);
 
+/* The same three with FP32 storage and float-float compensated arithmetic
+ * (FP32-compensated): B and D split once into hi + lo floats, every
+ * contraction, flux and coefficient product a Dot2 sum or TwoProd of exact
+ * floats and float-float values (the scheme of the reference's float-float
+ * BK1 kernels, openmp/BK1_ff.cpp:73-241, float_float.h:121-134, applied to
+ * the operator, the residual, the loss and both gradients); outputs joined
+ * to FP64.  Error vs the FP64 entry points on the same FP32 inputs ~1e-14
+ * relative. */
+int bkx_bk3_apply_kappa_f32c(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                             const float* kappa, const float* in, double* out, bkx_stream_t stream);
+int bkx_bk3_adjoint_f32c(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                         const float* kappa, const float* u, const float* b, double* dldu, double* dldkappa,
+                         double* loss, bkx_stream_t stream);
+int bkx_bk3_vjp_f32c(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                     const float* kappa, const float* u, const float* w, double* grad_u, double* grad_kappa,
+                     bkx_stream_t stream);
+
 /* *out = sum of x[0..n-1] in a fixed order (one workgroup; deterministic). */
 int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream);
 

@@ -38,6 +38,9 @@ _SIGNATURES = {
     "bkx_bk3_apply_kappa_f32w": [_c_int, _c_size] + [_vp] * 7,
     "bkx_bk3_adjoint_f32w": [_c_int, _c_size] + [_vp] * 10,
     "bkx_bk3_vjp_f32w": [_c_int, _c_size] + [_vp] * 9,
+    "bkx_bk3_apply_kappa_f32c": [_c_int, _c_size] + [_vp] * 7,
+    "bkx_bk3_adjoint_f32c": [_c_int, _c_size] + [_vp] * 10,
+    "bkx_bk3_vjp_f32c": [_c_int, _c_size] + [_vp] * 9,
     "bkx_sum_f64": [_vp, _c_size, _vp, _vp],
     "bkx_geo_bk3_f64": [_c_int, _c_size] + [_vp] * 7,
     "bkx_geo_bk5_f64": [_c_int, _c_size] + [_vp] * 5,

@@ -20,10 +20,14 @@
 // (bk3_phases.hpp).
 //
 // TS is the storage type of the inputs u, b (or w), kappa and G: double for
-// the FP64 entry points, float for the FP32-storage forms (_f32w), which
-// convert each value to FP64 when it is loaded and compute in FP64, so that
-// they apply the FP64 operator to the FP32 inputs exactly (dL/du, dL/dkappa
-// and the loss are FP64 in both).
+// the FP64 entry points, float for the FP32-storage forms, which compute
+// either in FP64 (_f32w: each value converted when it is loaded, so that the
+// FP64 operator is applied to the FP32 inputs exactly) or in float-float
+// (_f32c, FF set: the scheme of the reference's float-float BK1,
+// openmp/BK1_ff.cpp:73-241 -- B and D split once into hi + lo floats, every
+// contraction, flux and product a Dot2 compensated sum or TwoProd of
+// bkx_ff.hpp, the loaded floats exact).  dL/du, dL/dkappa and the loss are
+// FP64 in every form (the float-float ones joined with one add).
 #include "bk3_phases.hpp"
 #include "bkx_launch.hpp"
 
@@ -47,17 +51,93 @@ struct AdjShape {
     static constexpr int threads = ((epb * nq2 + kWave - 1) / kWave) * kWave;
 };
 
+// Per-point arithmetic of the kernel in FP64 (V double; L double, the type
+// a loaded value is kept in) or float-float (V ff; L float, exact).
+//   flux(a, b, c, g0, g1, g2) = g0 a + g1 b + g2 c
+//   scale(k, x)               = k x
+//   dot3(a, x, b, y, c, z)    = a x + b y + c z
+//   sub(o, b)                 = o - b
+__device__ __forceinline__ double flux(double a, double b, double c, double g0, double g1, double g2)
+{
+    return g0 * a + g1 * b + g2 * c;
+}
+__device__ __forceinline__ ff flux(ff a, ff b, ff c, float g0, float g1, float g2)
+{
+    dot2 d;
+    d.add(a, g0);
+    d.add(b, g1);
+    d.add(c, g2);
+    return d.value();
+}
+__device__ __forceinline__ double scale(double k, double x) { return k * x; }
+__device__ __forceinline__ ff scale(float k, ff x) { return ff_mul(x, k); }
+__device__ __forceinline__ double dot3(double a, double x, double b, double y, double c, double z)
+{
+    return a * x + b * y + c * z;
+}
+__device__ __forceinline__ ff dot3(ff a, ff x, ff b, ff y, ff c, ff z)
+{
+    dot2 d;
+    d.add(a, x);
+    d.add(b, y);
+    d.add(c, z);
+    return d.value();
+}
+__device__ __forceinline__ double sub(double o, double b) { return o - b; }
+__device__ __forceinline__ ff sub(ff o, float b)
+{
+    dot2 d;
+    d.add(o);
+    d.add(ff{-b, 0.0f});
+    return d.value();
+}
+__device__ __forceinline__ double as_f64(double x) { return x; }
+__device__ __forceinline__ double as_f64(ff x) { return ff_join(x); }
+__device__ __forceinline__ double as_v(double x) { return x; }
+__device__ __forceinline__ ff as_v(float x) { return ff{x, 0.0f}; }
+
+template <bool FF>
+struct AdjTypes {
+    using V = double;
+    using L = double;
+    using M = double;
+};
+template <>
+struct AdjTypes<true> {
+    using V = ff;
+    using L = float;
+    using M = ff;
+};
+
 // EO: Bm and Dm point at the even-odd forms of B and D (bkx_eo.hpp).
-template <int NM, int NQ, int MODE, typename TS, bool EO>
+// FF: float-float arithmetic (TS float); B and D are split into an LDS copy.
+template <int NM, int NQ, int MODE, typename TS, bool EO, bool FF = false>
 __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
-    size_t nelmt, const double* __restrict__ Bm, const double* __restrict__ Dm, const TS* __restrict__ G,
+    size_t nelmt, const double* __restrict__ Bg, const double* __restrict__ Dg, const TS* __restrict__ G,
     const TS* __restrict__ kappa, const TS* __restrict__ u, const TS* __restrict__ bw,
     double* __restrict__ out_u, double* __restrict__ out_k, double* __restrict__ loss)
 {
+    static_assert(!FF || (!EO && sizeof(TS) == 4), "float-float: FP32 storage, direct bases");
     using S = AdjShape<NM, NQ>;
-    using E = bk3p::Elem<NM, NQ>;
+    using V = typename AdjTypes<FF>::V;
+    using L = typename AdjTypes<FF>::L;
+    using MT = typename AdjTypes<FF>::M;
+    using E = bk3p::Elem<NM, NQ, V>;
     constexpr int nq2 = S::nq2;
-    __shared__ double s_box[S::epb][3 * E::box];
+    __shared__ V s_box[S::epb][3 * E::box];
+
+    const MT* Bm;
+    const MT* Dm;
+    if constexpr (FF) {
+        __shared__ ff s_bases[NM * NQ + NQ * NQ];
+        bk3p::split_bases<NM, NQ>(Bg, Dg, s_bases, s_bases + NM * NQ);
+        __syncthreads();
+        Bm = s_bases;
+        Dm = s_bases + NM * NQ;
+    } else {
+        Bm = Bg;
+        Dm = Dg;
+    }
 
     const int slot = threadIdx.x / nq2;
     const int l = threadIdx.x - slot * nq2;
@@ -75,7 +155,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
     // needed when a conditional block ends is waited for there, which for
     // FP32 storage serialised every load of the kernel.
     const int lm = modal ? l : 0;
-    auto modal_load = [&](const TS* base, double (&dst)[NM]) {
+    auto modal_load = [&](const TS* base, L (&dst)[NM]) {
         TS raw[NM];
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
@@ -83,20 +163,20 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
         }
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            dst[i] = (active && modal) ? double(raw[i]) : 0.0;
+            dst[i] = (active && modal) ? L(raw[i]) : L(0);
         }
     };
 
     // gradient of u
-    double x[NM];
+    L x[NM];
     modal_load(u, x);
-    double qr[NQ];
+    V qr[NQ];
     el.template grad<0, EO>(Bm, Dm, x, qr);
 
     // F = G P u (kept), kappa F -> rr (registers), rs -> B, rt -> C
     const TS* ge = G + ee * 6 * E::nq3 + l;      // G(e, f, p, q, r) at f = p = 0
     const TS* ke = kappa + ee * E::nq3 + l;      // kappa(e, p, q, r) at p = 0
-    double F0[NQ], F1[NQ], F2[NQ], rr[NQ];
+    V F0[NQ], F1[NQ], F2[NQ], rr[NQ];
     if (in_elem) {
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {
@@ -106,47 +186,63 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
                 graw[f] = ge[f * E::nq3 + p * nq2];
             }
             const TS kraw = ke[p * nq2];
-            double g[6];
+            L g[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                g[f] = active ? double(graw[f]) : 0.0;
+                g[f] = active ? L(graw[f]) : L(0);
             }
-            const double kap = active ? double(kraw) : 0.0;
+            const L kap = active ? L(kraw) : L(0);
             const int xx = E::X(p, el.lq, el.lq2);
-            const double qs = el.B[xx];
-            const double qt = el.C[xx];
-            F0[p] = g[0] * qr[p] + g[1] * qs + g[2] * qt;
-            F1[p] = g[1] * qr[p] + g[3] * qs + g[4] * qt;
-            F2[p] = g[2] * qr[p] + g[4] * qs + g[5] * qt;
-            rr[p] = kap * F0[p];
-            el.B[xx] = kap * F1[p];
-            el.C[xx] = kap * F2[p];
+            const V qs = el.B[xx];
+            const V qt = el.C[xx];
+            F0[p] = flux(qr[p], qs, qt, g[0], g[1], g[2]);
+            F1[p] = flux(qr[p], qs, qt, g[1], g[3], g[4]);
+            F2[p] = flux(qr[p], qs, qt, g[2], g[4], g[5]);
+            rr[p] = scale(kap, F0[p]);
+            el.B[xx] = scale(kap, F1[p]);
+            el.C[xx] = scale(kap, F2[p]);
         }
     }
     __syncthreads();
 
-    double r[NM];
+    V r[NM];
     if constexpr (MODE == kVjp) {
-        modal_load(bw, r);
+        L w[NM];
+        modal_load(bw, w);
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            r[i] = as_v(w[i]);
+        }
     } else {
-        double o[NM];
+        V o[NM];
         el.template div<10, EO>(Bm, Dm, rr, o);
         if constexpr (MODE == kApply) {
             if (active && modal) {
 #pragma unroll
                 for (int i = 0; i < NM; ++i) {
-                    out_u[e * E::nm3 + i * E::nm2 + l] = o[i];
+                    out_u[e * E::nm3 + i * E::nm2 + l] = as_f64(o[i]);
                 }
             }
             return;
         } else {
-            double bv[NM];
+            L bv[NM];
             modal_load(bw, bv);
-            double part = 0.0;
+            V part;
+            if constexpr (FF) {
+                dot2 acc;
 #pragma unroll
-            for (int i = 0; i < NM; ++i) {
-                r[i] = (active && modal) ? o[i] - bv[i] : 0.0;
-                part += r[i] * r[i];
+                for (int i = 0; i < NM; ++i) {
+                    r[i] = (active && modal) ? sub(o[i], bv[i]) : ff{0.0f, 0.0f};
+                    acc.add(r[i], r[i]);
+                }
+                part = acc.value();
+            } else {
+                part = 0.0;
+#pragma unroll
+                for (int i = 0; i < NM; ++i) {
+                    r[i] = (active && modal) ? sub(o[i], bv[i]) : 0.0;
+                    part += r[i] * r[i];
+                }
             }
             // 1/2 |r_e|^2: the element's lanes summed in lane order through box C
             // (free until the third phase of the next gradient)
@@ -155,18 +251,26 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
             }
             __syncthreads();
             if (active && l == 0) {
-                double s = 0.0;
-                for (int t = 0; t < E::nm2; ++t) {
-                    s += el.C[t];
+                if constexpr (FF) {
+                    dot2 s;
+                    for (int t = 0; t < E::nm2; ++t) {
+                        s.add(el.C[t]);
+                    }
+                    loss[e] = 0.5 * ff_join(s.value());
+                } else {
+                    double s = 0.0;
+                    for (int t = 0; t < E::nm2; ++t) {
+                        s += el.C[t];
+                    }
+                    loss[e] = 0.5 * s;
                 }
-                loss[e] = 0.5 * s;
             }
             __syncthreads();
         }
     }
 
     // gradient of r, dL/dkappa = P r . F, kappa G P r
-    double qrr[NQ];
+    V qrr[NQ];
     el.template grad<20, EO>(Bm, Dm, r, qrr);
     if (in_elem) {
 #pragma unroll
@@ -177,35 +281,35 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
                 graw[f] = ge[f * E::nq3 + p * nq2];
             }
             const TS kraw = ke[p * nq2];
-            double g[6];
+            L g[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                g[f] = active ? double(graw[f]) : 0.0;
+                g[f] = active ? L(graw[f]) : L(0);
             }
-            const double kap = active ? double(kraw) : 0.0;
+            const L kap = active ? L(kraw) : L(0);
             const int xx = E::X(p, el.lq, el.lq2);
-            const double qs = el.B[xx];
-            const double qt = el.C[xx];
+            const V qs = el.B[xx];
+            const V qt = el.C[xx];
             if (active) {
-                out_k[e * E::nq3 + p * nq2 + l] = qrr[p] * F0[p] + qs * F1[p] + qt * F2[p];
+                out_k[e * E::nq3 + p * nq2 + l] = as_f64(dot3(qrr[p], F0[p], qs, F1[p], qt, F2[p]));
             }
-            rr[p] = kap * (g[0] * qrr[p] + g[1] * qs + g[2] * qt);
-            el.B[xx] = kap * (g[1] * qrr[p] + g[3] * qs + g[4] * qt);
-            el.C[xx] = kap * (g[2] * qrr[p] + g[4] * qs + g[5] * qt);
+            rr[p] = scale(kap, flux(qrr[p], qs, qt, g[0], g[1], g[2]));
+            el.B[xx] = scale(kap, flux(qrr[p], qs, qt, g[1], g[3], g[4]));
+            el.C[xx] = scale(kap, flux(qrr[p], qs, qt, g[2], g[4], g[5]));
         }
     }
     __syncthreads();
-    double o2[NM];
+    V o2[NM];
     el.template div<30, EO>(Bm, Dm, rr, o2);
     if (active && modal) {
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            out_u[e * E::nm3 + i * E::nm2 + l] = o2[i];
+            out_u[e * E::nm3 + i * E::nm2 + l] = as_f64(o2[i]);
         }
     }
 }
 
-template <int NM, int MODE, typename TS, bool EO = false>
+template <int NM, int MODE, typename TS, bool EO = false, bool FF = false>
 int launch_adj(size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
                const TS* bw, double* out_u, double* out_k, double* loss, hipStream_t s)
 {
@@ -214,8 +318,8 @@ int launch_adj(size_t nelmt, const double* B, const double* D, const TS* G, cons
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL((bk3_adj_kernel<NM, NM + 1, MODE, TS, EO>), dim3(grid), dim3(S::threads), 0, s, nelmt, B, D, G,
-                       kappa, u, bw, out_u, out_k, loss);
+    hipLaunchKernelGGL((bk3_adj_kernel<NM, NM + 1, MODE, TS, EO, FF>), dim3(grid), dim3(S::threads), 0, s, nelmt, B,
+                       D, G, kappa, u, bw, out_u, out_k, loss);
     return launch_status();
 }
 
@@ -245,22 +349,23 @@ __global__ __launch_bounds__(kSumThreads) void sum_kernel(const double* __restri
 
 }  // namespace
 
-template <typename TS>
+template <typename TS, bool FF = false>
 int bk3_apply_kappa(int p, size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
                     double* v, hipStream_t s)
 {
     return with_order<1, 8>(p, [&](auto P) {
-        return launch_adj<decltype(P)::value + 1, kApply, TS>(nelmt, B, D, G, kappa, u, nullptr, v, nullptr, nullptr,
-                                                              s);
+        return launch_adj<decltype(P)::value + 1, kApply, TS, false, FF>(nelmt, B, D, G, kappa, u, nullptr, v,
+                                                                         nullptr, nullptr, s);
     });
 }
 
-template <typename TS>
+template <typename TS, bool FF = false>
 int bk3_adjoint(int p, size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
                 const TS* b, double* dldu, double* dldk, double* loss, hipStream_t s)
 {
     return with_order<1, 8>(p, [&](auto P) {
-        return launch_adj<decltype(P)::value + 1, kLoss, TS>(nelmt, B, D, G, kappa, u, b, dldu, dldk, loss, s);
+        return launch_adj<decltype(P)::value + 1, kLoss, TS, false, FF>(nelmt, B, D, G, kappa, u, b, dldu, dldk,
+                                                                        loss, s);
     });
 }
 
@@ -276,12 +381,13 @@ int bk3_adjoint_eo(int p, size_t nelmt, const double* P, const TS* G, const TS* 
     });
 }
 
-template <typename TS>
+template <typename TS, bool FF = false>
 int bk3_vjp(int p, size_t nelmt, const double* B, const double* D, const TS* G, const TS* kappa, const TS* u,
             const TS* w, double* gu, double* gk, hipStream_t s)
 {
     return with_order<1, 8>(p, [&](auto P) {
-        return launch_adj<decltype(P)::value + 1, kVjp, TS>(nelmt, B, D, G, kappa, u, w, gu, gk, nullptr, s);
+        return launch_adj<decltype(P)::value + 1, kVjp, TS, false, FF>(nelmt, B, D, G, kappa, u, w, gu, gk, nullptr,
+                                                                       s);
     });
 }
 
@@ -319,6 +425,24 @@ int bk3_vjp_f32w(int p, size_t nelmt, const double* B, const double* D, const fl
                  const float* u, const float* w, double* gu, double* gk, hipStream_t s)
 {
     return bk3_vjp(p, nelmt, B, D, G, kappa, u, w, gu, gk, s);
+}
+
+int bk3_apply_kappa_f32c(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                         const float* u, double* v, hipStream_t s)
+{
+    return bk3_apply_kappa<float, true>(p, nelmt, B, D, G, kappa, u, v, s);
+}
+
+int bk3_adjoint_f32c(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                     const float* u, const float* b, double* dldu, double* dldk, double* loss, hipStream_t s)
+{
+    return bk3_adjoint<float, true>(p, nelmt, B, D, G, kappa, u, b, dldu, dldk, loss, s);
+}
+
+int bk3_vjp_f32c(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                 const float* u, const float* w, double* gu, double* gk, hipStream_t s)
+{
+    return bk3_vjp<float, true>(p, nelmt, B, D, G, kappa, u, w, gu, gk, s);
 }
 
 int bk3_adjoint_eo_f64(int p, size_t nelmt, const double* P, const double* G, const double* kappa, const double* u,

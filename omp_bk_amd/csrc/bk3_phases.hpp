@@ -6,6 +6,11 @@
 // padded LDS boxes A, B, C of nq^3 values indexed X(p|i, q|j, r|k).  Between
 // calls the workgroup must be synchronised by the caller where noted.
 //
+// The value type V is double (FP64 arithmetic, B and D read as
+// wavefront-uniform scalar operands) or ff (float-float arithmetic: every
+// contraction a Dot2 compensated sum, bkx_ff.hpp, with B and D split into
+// float-float in a per-workgroup LDS copy).
+//
 //   grad(x):  modal values x[i] of lane (j,k) (l < nm^2), column i
 //             -> qr[p] in lane (q,r), column p; qs(p,q,r) in box B and
 //                qt(p,q,r) in box C (the reference gradient at the Gauss
@@ -18,6 +23,7 @@
 
 #include "bkx_device.hpp"
 #include "bkx_eo.hpp"
+#include "bkx_ff.hpp"
 
 namespace bkx {
 namespace bk3p {
@@ -99,6 +105,57 @@ __device__ __forceinline__ void rows3(P M, const double (&in0)[NIN], double (&ou
         out1[c] = t1;
         out2[c] = t2;
     }
+}
+
+// The float-float contractions: M is the float-float LDS copy of B or D,
+// the inputs are exact floats (loaded values) or float-float values, and
+// every output is a Dot2 sum over m ascending.
+template <int LD, int NIN, int NOUT, typename X>
+__device__ __forceinline__ void cols(const ff* M, const X (&in)[NIN], ff (&out)[NOUT])
+{
+    dot2 acc[NOUT];
+#pragma unroll
+    for (int m = 0; m < NIN; ++m) {
+#pragma unroll
+        for (int c = 0; c < NOUT; ++c) {
+            acc[c].add(M[m * LD + c], in[m]);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) {
+        out[c] = acc[c].value();
+    }
+}
+
+template <int LD, int NIN, int NOUT>
+__device__ __forceinline__ void cols2(const ff* M, const ff (&in0)[NIN], ff (&out0)[NOUT], const ff (&in1)[NIN],
+                                      ff (&out1)[NOUT])
+{
+    cols<LD>(M, in0, out0);
+    cols<LD>(M, in1, out1);
+}
+
+template <int LD, int NIN, int NOUT>
+__device__ __forceinline__ void rows(const ff* M, const ff (&in)[NIN], ff (&out)[NOUT])
+{
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) {
+        dot2 t;
+#pragma unroll
+        for (int m = 0; m < NIN; ++m) {
+            t.add(M[c * LD + m], in[m]);
+        }
+        out[c] = t.value();
+    }
+}
+
+template <int LD, int NIN, int NOUT>
+__device__ __forceinline__ void rows3(const ff* M, const ff (&in0)[NIN], ff (&out0)[NOUT], const ff (&in1)[NIN],
+                                      ff (&out1)[NOUT], const ff (&in2)[NIN], ff (&out2)[NOUT])
+{
+    rows<LD>(M, in0, out0);
+    rows<LD>(M, in1, out1);
+    rows<LD>(M, in2, out2);
 }
 
 // The same contractions with the matrix in even-odd form (bkx_eo.hpp).
@@ -218,8 +275,45 @@ __device__ __forceinline__ auto op_d(const double* p)
         return launder<TAG>(p);
     }
 }
+// the float-float LDS copies (no even-odd form)
+template <int TAG, bool EO, int NM, int NQ>
+__device__ __forceinline__ const ff* op_b(const ff* p)
+{
+    static_assert(!EO, "the float-float contractions use B directly");
+    return p;
+}
+template <int TAG, bool EO, int NQ>
+__device__ __forceinline__ const ff* op_d(const ff* p)
+{
+    static_assert(!EO, "the float-float contractions use D directly");
+    return p;
+}
 
+// B (nm x nq) and D (nq x nq) split into float-float into the LDS arrays sB,
+// sD by the workgroup's threads; the caller synchronises afterwards.
 template <int NM, int NQ>
+__device__ __forceinline__ void split_bases(const double* __restrict__ B, const double* __restrict__ D, ff* sB, ff* sD)
+{
+    for (int x = threadIdx.x; x < NM * NQ; x += blockDim.x) {
+        sB[x] = ff_split(B[x]);
+    }
+    for (int x = threadIdx.x; x < NQ * NQ; x += blockDim.x) {
+        sD[x] = ff_split(D[x]);
+    }
+}
+
+// a + b + c, left to right (FP64) or as one compensated sum (float-float)
+__device__ __forceinline__ double add3(double a, double b, double c) { return a + b + c; }
+__device__ __forceinline__ ff add3(ff a, ff b, ff c)
+{
+    dot2 d;
+    d.add(a);
+    d.add(b);
+    d.add(c);
+    return d.value();
+}
+
+template <int NM, int NQ, typename V = double>
 struct Elem {
     static constexpr int nm2 = NM * NM;
     static constexpr int nm3 = nm2 * NM;
@@ -228,9 +322,9 @@ struct Elem {
     static constexpr int row = NQ | 1;            // odd row stride in 8-byte words (bank spread)
     static constexpr int box = NQ * NQ * row;
 
-    double* A;
-    double* B;
-    double* C;
+    V* A;
+    V* B;
+    V* C;
     int l;            // lane within the element's plane, 0 .. nq^2 - 1
     bool in_elem;     // lane belongs to an element slot of the workgroup
     int lm, lm2;      // (l / NM, l % NM)
@@ -238,7 +332,7 @@ struct Elem {
 
     __device__ static int X(int a, int b, int c) { return (a * NQ + b) * row + c; }
 
-    __device__ void init(double* boxes, int lane, bool in)
+    __device__ void init(V* boxes, int lane, bool in)
     {
         A = boxes;
         B = boxes + box;
@@ -252,12 +346,14 @@ struct Elem {
     }
 
     // modal x (lane (j,k), l < nm^2) -> qr (lane (q,r)), qs -> B, qt -> C.
-    // Ends with the workgroup synchronised.
-    template <int TAG, bool EO = false>
-    __device__ void grad(const double* Bm, const double* Dm, const double (&x)[NM], double (&qr)[NQ])
+    // Ends with the workgroup synchronised.  MT: double (device rows of B
+    // and D) or ff (their LDS copy); XT: V or, for ff, the exact float
+    // values as loaded.
+    template <int TAG, bool EO = false, typename MT, typename XT>
+    __device__ void grad(const MT* Bm, const MT* Dm, const XT (&x)[NM], V (&qr)[NQ])
     {
         if (in_elem && l < nm2) {
-            double a[NQ];
+            V a[NQ];
             cols<NQ>(op_b<TAG + 1, EO, NM, NQ>(Bm), x, a);
 #pragma unroll
             for (int p = 0; p < NQ; ++p) {
@@ -266,7 +362,7 @@ struct Elem {
         }
         __syncthreads();
         if (in_elem && l < NQ * NM) {
-            double y[NM], b[NQ];
+            V y[NM], b[NQ];
 #pragma unroll
             for (int j = 0; j < NM; ++j) {
                 y[j] = A[X(lm, j, lm2)];
@@ -279,7 +375,7 @@ struct Elem {
         }
         __syncthreads();
         if (in_elem) {
-            double y[NM], U[NQ], qt[NQ];
+            V y[NM], U[NQ], qt[NQ];
 #pragma unroll
             for (int k = 0; k < NM; ++k) {
                 y[k] = B[X(lq, lq2, k)];
@@ -294,7 +390,7 @@ struct Elem {
         }
         __syncthreads();
         if (in_elem) {
-            double y[NQ], z[NQ], qs[NQ];
+            V y[NQ], z[NQ], qs[NQ];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 y[q] = A[X(lq, q, lq2)];
@@ -312,12 +408,12 @@ struct Elem {
     // rr (lane (q,r)), rs in B, rt in C -> o (lane (j,k), l < nm^2).
     // Starts after a workgroup synchronisation (the caller's writes of B, C);
     // ends with the workgroup synchronised.
-    template <int TAG, bool EO = false>
-    __device__ void div(const double* Bm, const double* Dm, const double (&rr)[NQ], double (&o)[NM])
+    template <int TAG, bool EO = false, typename MT>
+    __device__ void div(const MT* Bm, const MT* Dm, const V (&rr)[NQ], V (&o)[NM])
     {
-        double Ws[NQ], Wt[NQ];
+        V Ws[NQ], Wt[NQ];
         if (in_elem) {
-            double Wr[NQ], y[NQ], z[NQ];
+            V Wr[NQ], y[NQ], z[NQ];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 y[q] = B[X(lq, q, lq2)];
@@ -338,11 +434,11 @@ struct Elem {
         }
         __syncthreads();
         if (in_elem) {
-            double W[NQ], Y[NM];
+            V W[NQ], Y[NM];
 #pragma unroll
             for (int r = 0; r < NQ; ++r) {
                 const int x = X(lq, lq2, r);
-                W[r] = Wt[r] + A[x] + B[x];
+                W[r] = add3(Wt[r], A[x], B[x]);
             }
             rows<NQ>(op_b<TAG + 2, EO, NM, NQ>(Bm), W, Y);
 #pragma unroll
@@ -352,7 +448,7 @@ struct Elem {
         }
         __syncthreads();
         if (in_elem && l < NQ * NM) {
-            double y[NQ], Z[NM];
+            V y[NQ], Z[NM];
 #pragma unroll
             for (int q = 0; q < NQ; ++q) {
                 y[q] = A[X(lm, q, lm2)];
@@ -365,7 +461,7 @@ struct Elem {
         }
         __syncthreads();
         if (in_elem && l < nm2) {
-            double z[NQ];
+            V z[NQ];
 #pragma unroll
             for (int p = 0; p < NQ; ++p) {
                 z[p] = B[X(p, lm, lm2)];

@@ -31,6 +31,12 @@ int bk3_adjoint_f32w(int p, size_t nelmt, const double* B, const double* D, cons
                      const float* u, const float* b, double* dldu, double* dldk, double* loss, hipStream_t s);
 int bk3_vjp_f32w(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
                  const float* u, const float* w, double* gu, double* gk, hipStream_t s);
+int bk3_apply_kappa_f32c(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                         const float* u, double* v, hipStream_t s);
+int bk3_adjoint_f32c(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                     const float* u, const float* b, double* dldu, double* dldk, double* loss, hipStream_t s);
+int bk3_vjp_f32c(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
+                 const float* u, const float* w, double* gu, double* gk, hipStream_t s);
 int sum_f64(const double* x, size_t n, double* out, hipStream_t s);
 int geo_bk3_f64(int p, size_t nelmt, const double* B, const double* D, const double* wq, const double* coords,
                 double* G, double* JxW, hipStream_t s);
@@ -594,6 +600,53 @@ int bkx_bk3_vjp_f32w(int p, size_t nelmt, const double* basis, const double* dba
         return BKX_ERROR_NULL;
     }
     return bkx::bk3_vjp_f32w(p, nelmt, basis, dbasis, G, kappa, u, w, grad_u, grad_kappa, as_stream(stream));
+}
+
+int bkx_bk3_apply_kappa_f32c(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                             const float* kappa, const float* in, double* out, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !kappa || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_apply_kappa_f32c(p, nelmt, basis, dbasis, G, kappa, in, out, as_stream(stream));
+}
+
+int bkx_bk3_adjoint_f32c(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                         const float* kappa, const float* u, const float* b, double* dldu, double* dldkappa,
+                         double* loss, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !kappa || !u || !b || !dldu || !dldkappa || !loss) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_adjoint_f32c(p, nelmt, basis, dbasis, G, kappa, u, b, dldu, dldkappa, loss, as_stream(stream));
+}
+
+int bkx_bk3_vjp_f32c(int p, size_t nelmt, const double* basis, const double* dbasis, const float* G,
+                     const float* kappa, const float* u, const float* w, double* grad_u, double* grad_kappa,
+                     bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!basis || !dbasis || !G || !kappa || !u || !w || !grad_u || !grad_kappa) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk3_vjp_f32c(p, nelmt, basis, dbasis, G, kappa, u, w, grad_u, grad_kappa, as_stream(stream));
 }
 
 int bkx_geo_bk3_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* qweights,

@@ -223,7 +223,7 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
     return out
 
 
-_KAPPA_FORMS = {"f64": torch.float64, "f32w": torch.float32}
+_KAPPA_FORMS = {"f64": torch.float64, "f32w": torch.float32, "f32c": torch.float32}
 
 
 def _bk3_common(p, basis, dbasis, G, u, form="f64"):
@@ -245,7 +245,8 @@ def _bk3_common(p, basis, dbasis, G, u, form="f64"):
 def bk3_kappa(p, basis, dbasis, G, kappa, u, out=None, form="f64"):
     """v = A(kappa) u: BK3 with each Gauss point's six factors scaled by
     kappa(e, p, q, r).  form "f64": every array FP64; "f32w": u, G and kappa
-    stored in FP32, FP64 arithmetic; v FP64 in both."""
+    stored in FP32, FP64 arithmetic; "f32c": the same FP32 storage with
+    float-float compensated arithmetic; v FP64 in every form."""
     ne, nm, nq, sdt = _bk3_common(p, basis, dbasis, G, u, form)
     _check("kappa", kappa, ne * nq ** 3, sdt)
     if out is None:

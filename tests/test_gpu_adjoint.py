@@ -170,3 +170,77 @@ def test_f32w_full_size_matches_f64_on_same_inputs():
     assert abs(L32.item() - L64.item()) <= 1e-13 * L64.item()
     assert rel(dldu32, dldu64) <= 1e-13
     assert rel(dldk32, dldk64) <= 1e-13
+
+
+@pytest.mark.parametrize("p", range(1, 9))
+def test_f32c_against_oracle(p):
+    """FP32-compensated forms (float-float arithmetic on FP32 storage)
+    against the FP64 oracle on the same FP32-rounded inputs: bound 1e-12
+    relative (the float-float sums carry about 44 significant bits; two
+    operator applications and the products of dL/dkappa stay within it)."""
+    E = 7
+    B, D, G, kappa, u, b = _host_case(p, E, seed=90 + p)
+    G, kappa, u, b = (x.astype(np.float32) for x in (G, kappa, u, b))
+    d = lambda x: x.astype(np.float64)
+    f = lambda x: _t(x).float()
+    nq = p + 2
+    Lr, r, dldu_r, dldk_r = O.bk3_adjoint(nq, B, D, d(G), d(kappa), d(u), d(b))
+    v = ops.bk3_kappa(p, _t(B), _t(D), f(G), f(kappa), f(u), form="f32c").cpu().numpy()
+    assert O.rel_maxdiff(v, O.bk3_kappa(nq, B, D, d(G), d(kappa), d(u))) <= 1e-12
+    L, dldu, dldk, loss_e = ops.bk3_adjoint(p, _t(B), _t(D), f(G), f(kappa), f(u), f(b), form="f32c")
+    assert dldu.dtype == torch.float64 and dldk.dtype == torch.float64
+    assert abs(L.item() - Lr) <= 1e-12 * Lr
+    assert O.rel_maxdiff(dldu.cpu().numpy(), dldu_r) <= 1e-12
+    assert O.rel_maxdiff(dldk.cpu().numpy(), dldk_r) <= 1e-12
+    w = np.random.default_rng(p).uniform(-1, 1, u.size).astype(np.float32)
+    _, _, gu_r, gk_r = O.bk3_adjoint(nq, B, D, d(G), d(kappa), d(u), O.bk3_kappa(nq, B, D, d(G), d(kappa), d(u)) - d(w))
+    gu, gk = ops.bk3_vjp(p, _t(B), _t(D), f(G), f(kappa), f(u), f(w), form="f32c")
+    assert O.rel_maxdiff(gu.cpu().numpy(), gu_r) <= 1e-12
+    assert O.rel_maxdiff(gk.cpu().numpy(), gk_r) <= 1e-12
+
+
+@pytest.mark.parametrize("p,E", [(1, 1000), (4, 333), (8, 97)])
+def test_f32c_ragged_matches_f64(p, E):
+    """Ragged element counts (partial last workgroup) in the f32c form
+    against the FP64 entry points on the same FP32-rounded inputs."""
+    B, D, G, kappa, u, b = _host_case(p, E, seed=3 * p + E)
+    f = lambda x: _t(x.astype(np.float32))
+    G32, k32, u32, b32 = f(G), f(kappa), f(u), f(b)
+    L64, dldu64, dldk64, le64 = ops.bk3_adjoint(p, _t(B), _t(D), G32.double(), k32.double(), u32.double(), b32.double())
+    L32, dldu32, dldk32, le32 = ops.bk3_adjoint(p, _t(B), _t(D), G32, k32, u32, b32, form="f32c")
+    rel = lambda x, r: float(((x - r).abs().max() / r.abs().max()).item())
+    assert abs(L32.item() - L64.item()) <= 1e-12 * L64.item()
+    assert rel(le32, le64) <= 1e-12
+    assert rel(dldu32, dldu64) <= 1e-12
+    assert rel(dldk32, dldk64) <= 1e-12
+
+
+def test_f32c_full_size_finite_differences():
+    """The adjoint configuration (p = 6, E = 16384) in the FP32-compensated
+    form: its gradients, on FP32-rounded inputs, against central
+    differences (h = 1e-6, 64 random directions each for u and kappa) of
+    the FP64 loss at the same point, and against the FP64 entry point's
+    gradients to 1e-12.  The perturbed points are not FP32 values, so the
+    differences are taken of the FP64 per-element losses."""
+    p, E = 6, 16384
+    B, D, G, kappa, u, b, g = _device_case(p, E, 13)
+    G, kappa, u, b = (x.float() for x in (G, kappa, u, b))
+    L, dldu, dldk, loss_e = ops.bk3_adjoint(p, B, D, G, kappa, u, b, form="f32c")
+    Gd, kd, ud, bd = (x.double() for x in (G, kappa, u, b))
+    L64, dldu64, dldk64, _ = ops.bk3_adjoint(p, B, D, Gd, kd, ud, bd)
+    rel = lambda x, r: float(((x - r).abs().max() / r.abs().max()).item())
+    assert abs(L.item() - L64.item()) <= 1e-12 * L64.item()
+    assert rel(dldu, dldu64) <= 1e-12
+    assert rel(dldk, dldk64) <= 1e-12
+    per_elem = lambda uu, kk: ops.bk3_adjoint(p, B, D, Gd, kk, uu, bd)[3]
+    h = 1e-6
+    tol = 8 * 2.2e-16 * loss_e.abs().max().item() * E ** 0.5 / (2 * h)
+    for t in range(64):
+        du = torch.randn(u.shape, dtype=torch.float64, device=DEV, generator=g)
+        dk = torch.randn(kappa.shape, dtype=torch.float64, device=DEV, generator=g)
+        fd_u = torch.sum(per_elem(ud + h * du, kd) - per_elem(ud - h * du, kd)).item() / (2 * h)
+        fd_k = torch.sum(per_elem(ud, kd + h * dk) - per_elem(ud, kd - h * dk)).item() / (2 * h)
+        an_u = torch.dot(dldu, du).item()
+        an_k = torch.dot(dldk, dk).item()
+        assert abs(fd_u - an_u) <= tol + 1e-9 * abs(an_u), (t, fd_u, an_u, tol)
+        assert abs(fd_k - an_k) <= tol + 1e-9 * abs(an_k), (t, fd_k, an_k, tol)

@@ -113,6 +113,16 @@ def test_real_geometry(kind, p):
     assert relerr(run(kind, p, *case), oracle(kind, p, *case)) <= TOL
 
 
+def test_bk1_config_whole_mesh():
+    """The BK1 workload itself: p = 3 (n = 4, q = 5), E = 4096 elements of the
+    16x16x16 perturbed box mesh, JxW = w detJ in FP64, u uniform[-1, 1] from
+    a seed; every element against the oracle (no sampling: 0.26 M DoF)."""
+    p, E = 3, 4096
+    assert geo.box_shape(E) == (16, 16, 16)
+    case = geo_case(1, p, E, seed=4096)
+    assert relerr(run(1, p, *case), oracle(1, p, *case)) <= TOL
+
+
 def test_empty_and_errors():
     z = torch.zeros(0, dtype=torch.float64, device=DEV)
     D = dev(geo.bk5_basis(3))

@@ -112,3 +112,29 @@ def test_bf16s_rejects_other_orders():
     with pytest.raises(Exception):
         ops.bk5(6, torch.zeros(49, dtype=torch.float64, device=DEV), torch.zeros(6 * 343, device=DEV),
                 torch.zeros(343, device=DEV), form="bf16s")
+
+
+def test_bf16s_full_size_sampled():
+    """The split-BF16 MFMA form on the full BK5 workload (p = 7, E = 65536,
+    64x32x32 perturbed mesh, u = U[-1, 1] + sin(pi x) sin(pi y) sin(pi z)):
+    sampled elements against the FP64 oracle on the same FP32 inputs, within
+    the form's 1e-4 bound; every output finite."""
+    p, E = 7, 65536
+    n = p + 1
+    D = torch.as_tensor(geo.bk5_basis(p), device=DEV)
+    G, xyz = geo.geometric_factors(5, p, E, device=DEV, coords=True)
+    G = G.reshape(-1)
+    torch.manual_seed(5)
+    u = (torch.rand(E * n ** 3, dtype=torch.float64, device=DEV) * 2 - 1) + geo.smooth_mode(xyz).reshape(-1)
+    del xyz
+    G32, u32 = G.float(), u.float()
+    del G, u
+    out = torch.full((E * n ** 3,), float("nan"), dtype=torch.float32, device=DEV)
+    ops.bk5(p, D, G32, u32, out=out, form="bf16s")
+    assert bool(torch.isfinite(out).all().item())
+    idx = torch.as_tensor(np.random.default_rng(6).choice(E, 256, replace=False), device=DEV)
+    ref = O.bk5(n, D.cpu().numpy(), G32.view(E, -1)[idx].double().cpu().numpy(),
+                u32.view(E, -1)[idx].double().cpu().numpy())
+    got = out.view(E, -1)[idx].double().cpu().numpy().reshape(-1)
+    err = O.rel_maxdiff(got, ref)
+    assert 1e-9 < err < 1e-4, err
