@@ -15,7 +15,6 @@ factors (openmp/BK5.cpp:143-144, openmp/BK3.cpp:50-51).
 Prints one JSON line.  python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -33,7 +32,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk5_bf16s", "bk3", "copy", "mixed"],
+    ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk5_bf16s", "bk3", "copy", "mixed",
+                                       "adjoint_f64", "adjoint_f32w", "adjoint_f32c", "adjoint_f64_eo",
+                                       "adjoint_f32w_eo"],
                     default=None,
                     help="run one workload only (for profiling) and print its timing")
     return ap.parse_args()
@@ -282,6 +283,29 @@ def rate(w, ms):
             "bytes_per_dof": round(w["bytes"] / w["dofs"], 3)}
 
 
+def adjoint_inputs(p=6, E=16384):
+    """The adjoint workload's inputs on cuda:0: bases, G of the perturbed
+    32x32x16 box mesh, kappa log-normal (sigma = 0.5), u and b U[-1, 1]."""
+    import torch
+    from omp_bk_amd import geometry as geo
+    dev = "cuda:0"
+    n, q = p + 1, p + 2
+    g = torch.Generator(device=dev).manual_seed(2024)
+    B, D = (torch.as_tensor(a, device=dev) for a in geo.bk13_bases(p))
+    G = geo.geometric_factors(3, p, E, device=dev).reshape(-1)
+    kappa = torch.exp(0.5 * torch.randn(E * q ** 3, dtype=torch.float64, device=dev, generator=g))
+    u = torch.rand(E * n ** 3, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    b = torch.rand(E * n ** 3, dtype=torch.float64, device=dev, generator=g) * 2 - 1
+    return B, D, G, kappa, u, b
+
+
+def adjoint_bytes(p, E, storage):
+    """Algorithmic bytes of one adjoint launch: u, b, kappa and G read once
+    in the storage type, dL/du and dL/dkappa written in FP64."""
+    n, q = p + 1, p + 2
+    return storage * (2 * E * n ** 3 + 7 * E * q ** 3) + 8 * (E * n ** 3 + E * q ** 3)
+
+
 def adjoint_config(steps, warmup, p=6, E=16384, forms=("f64", "f32w"), ms64=None):
     """The BK3 adjoint configuration: L = 1/2 |A(kappa) u - b|^2 with kappa
     log-normal (sigma = 0.5), u, b = U[-1, 1], p = 6, E = 16384; one step is
@@ -291,17 +315,9 @@ def adjoint_config(steps, warmup, p=6, E=16384, forms=("f64", "f32w"), ms64=None
     errors against FP64 on the same rounded inputs.  Bytes
     per DoF count every input once and both outputs (G and kappa are read a
     second time from the L2 cache)."""
-    import torch
-    from omp_bk_amd import geometry as geo
     from omp_bk_amd import ops
-    dev = "cuda:0"
+    B, D, G, kappa, u, b = adjoint_inputs(p, E)
     n, q = p + 1, p + 2
-    g = torch.Generator(device=dev).manual_seed(2024)
-    B, D = (torch.as_tensor(a, device=dev) for a in geo.bk13_bases(p))
-    G = geo.geometric_factors(3, p, E, device=dev).reshape(-1)
-    kappa = torch.exp(0.5 * torch.randn(E * q ** 3, dtype=torch.float64, device=dev, generator=g))
-    u = torch.rand(E * n ** 3, dtype=torch.float64, device=dev, generator=g) * 2 - 1
-    b = torch.rand(E * n ** 3, dtype=torch.float64, device=dev, generator=g) * 2 - 1
     dofs = E * n ** 3
     out = {}
     ref = None
@@ -314,7 +330,7 @@ def adjoint_config(steps, warmup, p=6, E=16384, forms=("f64", "f32w"), ms64=None
         res = {}
         fn = lambda: res.__setitem__("r", ops.bk3_adjoint(p, B, D, Gs, ks, us, bs, form=form))
         ms, _ = time_steps(fn, steps, warmup)
-        nbytes = s * (2 * dofs + 7 * E * q ** 3) + 8 * (dofs + E * q ** 3)
+        nbytes = adjoint_bytes(p, E, s)
         line = {"value": round(dofs / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
                 "bytes_per_dof": round(nbytes / dofs, 2), "achieved_GBs": round(nbytes / (ms * 1e-3) * 1e-9, 1),
                 "arithmetic": {"f64": "FP64", "f32w": "FP64 (FP32 storage)",
@@ -455,74 +471,42 @@ def probe_roofs(nbytes=2 << 30, steps=20):
     return out
 
 
-_ONE_CORE = r"""
-import ctypes, sys, time
-import numpy as np
-so, fn, arg0, n, ne = sys.argv[1], sys.argv[2], int(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5])
-f = getattr(ctypes.CDLL(so), fn)
-f.argtypes = [ctypes.c_int, ctypes.c_size_t] + [ctypes.c_void_p] * 4
-rng = np.random.default_rng(0)
-D, G = rng.uniform(-1, 1, n * n), rng.uniform(0.5, 1.5, ne * 6 * n ** 3)
-u = rng.uniform(-1, 1, ne * n ** 3)
-v = np.empty_like(u)
-best = float("inf")
-for _ in range(3):
-    t0 = time.perf_counter()
-    f(arg0, ne, D.ctypes.data, G.ctypes.data, u.ctypes.data, v.ctypes.data)
-    best = min(best, time.perf_counter() - t0)
-print(best)
-"""
-
-
-def cpu_baseline(w, sample_elems=4096, budget_s=10.0):
-    """BK5 FP64 on the host: the reference's own SumFactorization<double, 8>
-    (oracle/_ref, compiled from the reference sources with g++ -O3 -fopenmp)
-    when present, else the oracle's restatement; min time over repeated
-    applies of a sample of the workload's elements.  cores is the number of
-    cores the applies kept busy (process CPU time over wall time: the
-    reference's `target teams loop` runs as host teams, whose count
-    OMP_NUM_THREADS does not set), and value_1thread the rate of the same
-    kernel in a process pinned to one core."""
-    import numpy as np
+def cpu_baseline(budget_s=3.0):
+    """The reference's own CPU operators, SumFactorization<double, nq> for
+    BK1 p = 3 (E = 4096), BK3 p = 5 (E = 32768) and BK5 p = 7 (E = 65536),
+    compiled from the reference sources with g++ -O3 -fopenmp (oracle/_ref),
+    timed at the full workload sizes by oracle/cpu_bench.py: min over
+    repeated applies, once with every core of this process and once in a
+    process pinned to one core.  The headline fields are BK5's; cores is the
+    number of cores the applies kept busy (the reference's `target teams
+    loop` runs as host teams, whose count OMP_NUM_THREADS does not set)."""
     import subprocess
-    import sys
-    p, E = w["p"], w["E"]
-    n = p + 1
-    ne = min(sample_elems, E)
-    D = np.ascontiguousarray(w["D"].cpu().numpy())
-    G = np.ascontiguousarray(w["G"][: ne * 6 * n ** 3].cpu().numpy())
-    u = np.ascontiguousarray(w["u"][: ne * n ** 3].cpu().numpy())
-    v = np.empty_like(u)
-    ref_so = os.path.join(ROOT, "oracle", "_ref", "libbkref.so")
-    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
-    if os.path.exists(ref_so):
-        so, kind, fn, arg0 = ref_so, "reference", "bkref_bk5_sumfact", p
-    elif os.path.exists(port_so):
-        so, kind, fn, arg0 = port_so, "port", "bko_bk5_sumfact", n
-    else:
+    script = os.path.join(ROOT, "oracle", "cpu_bench.py")
+    if not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "libbkref.so")):
         return None
-    f = getattr(ctypes.CDLL(so), fn)
-    f.argtypes = [ctypes.c_int, ctypes.c_size_t] + [ctypes.c_void_p] * 4
-    call = lambda: f(arg0, ne, D.ctypes.data, G.ctypes.data, u.ctypes.data, v.ctypes.data)
-    best, t_start, reps = float("inf"), time.perf_counter(), 0
-    wall0, cpu0 = time.perf_counter(), time.process_time()
-    while reps < 3 or (time.perf_counter() - t_start < budget_s and reps < 50):
-        t0 = time.perf_counter()
-        call()
-        best = min(best, time.perf_counter() - t0)
-        reps += 1
-    busy = (time.process_time() - cpu0) / (time.perf_counter() - wall0)
-    out = {"value": round(ne * n ** 3 / best * 1e-9, 4), "unit": "GDoF/s", "cores": max(1, int(round(busy))),
-           "kind": kind, "sample": "BK5 p=%d FP64, first %d of the %d elements, min of %d applies" % (p, ne, E, reps),
-           "cores_available": len(os.sched_getaffinity(0))}
-    try:
-        core = min(os.sched_getaffinity(0))
-        r = subprocess.run([sys.executable, "-c", _ONE_CORE, so, fn, str(arg0), str(n), str(ne)], capture_output=True,
-                           text=True, timeout=120, preexec_fn=lambda: os.sched_setaffinity(0, {core}))
-        out["value_1thread"] = round(ne * n ** 3 / float(r.stdout.split()[-1]) * 1e-9, 4)
-    except (OSError, ValueError, IndexError, subprocess.SubprocessError):
-        pass
-    return out
+    runs = {}
+    for mode, extra in (("all", []), ("one", ["--one-core", "--budget", "2"])):
+        try:
+            r = subprocess.run([sys.executable, script] + extra + ["--budget", str(budget_s), "bk1", "bk3", "bk5"],
+                               capture_output=True, text=True, timeout=300, check=True)
+            runs[mode] = json.loads(r.stdout.strip().splitlines()[-1])
+        except (OSError, ValueError, IndexError, subprocess.SubprocessError) as exc:
+            print("cpu baseline (%s cores) failed: %s" % (mode, exc), file=sys.stderr)
+            return None
+    al, one = runs["all"], runs["one"]
+    b5 = al["configs"]["bk5"]
+    configs = {}
+    for name in ("bk1", "bk3", "bk5"):
+        a, o = al["configs"][name], one["configs"][name]
+        configs["%s_p%d" % (name, a["p"])] = {"nelmt": a["nelmt"], "GDoF/s": a["GDoF/s"], "ms": a["ms"],
+                                               "cores_busy": a["cores_busy"], "GDoF/s_1core": o["GDoF/s"],
+                                               "ms_1core": o["ms"]}
+    return {"value": b5["GDoF/s"], "unit": "GDoF/s", "cores": max(1, int(round(b5["cores_busy"]))),
+            "kind": "reference",
+            "sample": "whole workload, no sampling: BK5 p=7 E=65536 (and BK1 p=3 E=4096, BK3 p=5 E=32768), "
+                      "min of %d applies" % b5["applies"],
+            "value_1thread": one["configs"]["bk5"]["GDoF/s"], "cpu_model": al["cpu"]["model"],
+            "cores_available": al["cpu"]["cpus_available"], "isa": al["cpu"]["isa"], "configs": configs}
 
 
 def traffic(name):
@@ -593,6 +577,19 @@ def run_only(a):
     elif a.only == "bk3":
         w = bk3_workload(7, 65536)
         fn, nbytes = w["fn"], w["bytes"]
+    elif a.only.startswith("adjoint"):
+        p, E = 6, 16384
+        form = a.only.split("_")[1]
+        eo = a.only.endswith("_eo")
+        B, D, G, kappa, u, b = adjoint_inputs(p, E)
+        if form != "f64":
+            G, kappa, u, b = G.float(), kappa.float(), u.float(), b.float()
+        if eo:
+            P = ops.bk3_basis(p, B, D)
+            fn = lambda: ops.bk3_adjoint_prepared(p, P, G, kappa, u, b, form=form)
+        else:
+            fn = lambda: ops.bk3_adjoint(p, B, D, G, kappa, u, b, form=form)
+        nbytes = adjoint_bytes(p, E, 8 if form == "f64" else 4)
     else:
         w = bk5_workload()
         fn, nbytes = w["fn"], w["bytes"]
@@ -658,7 +655,7 @@ def main():
     lines.update(adjoint_config(a.steps, a.warmup, forms=("f32c",),
                                 ms64=lines["bk3_adjoint_p6_f64"]["ms_per_step"]))
 
-    cpu = None if a.no_cpu else cpu_baseline(w5)
+    cpu = None if a.no_cpu else cpu_baseline()
 
     line = {
         "metric": "BK3/BK5 GDoF/s at p=7, FP64 vs FP32-compensated at <=1e-12 rel err; % HBM roof",
