@@ -25,6 +25,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# What each reduced-precision form computes in (the bench lines carry it):
+# only f32c is FP32-compensated; f32w is the FP64 operator on FP32 inputs.
+ARITHMETIC = {
+    "f32w": "FP64 arithmetic on FP32 storage (not compensated)",
+    "f32w_eo": "FP64 arithmetic on FP32 storage, even-odd bases (not compensated)",
+    "f32c": "FP32-compensated: float-float (Dot2) arithmetic on FP32 storage",
+    "bf16s": "split-BF16 MFMA products accumulated in FP32 (reduced precision)",
+}
+
 
 def parse():
     ap = argparse.ArgumentParser()
@@ -167,6 +176,7 @@ def bk3_reduced(w, steps, warmup, ms64):
         ms, _ = time_steps(fn, steps, max(warmup, 20))
         out["bk3_p%d_%s" % (p, form)] = {
             "value": round(w["dofs"] / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
+            "arithmetic": ARITHMETIC[form],
             "speedup_vs_f64": round(ms64 / ms, 3), "bytes_per_dof": round(nb, 3),
             "frac": round(nb * w["dofs"] / (ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
             "relerr_vs_f64_same_inputs": float(((v - ref_q).abs().max() / ref_q.abs().max()).item()),
@@ -333,8 +343,7 @@ def adjoint_config(steps, warmup, p=6, E=16384, forms=("f64", "f32w"), ms64=None
         nbytes = adjoint_bytes(p, E, s)
         line = {"value": round(dofs / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
                 "bytes_per_dof": round(nbytes / dofs, 2), "achieved_GBs": round(nbytes / (ms * 1e-3) * 1e-9, 1),
-                "arithmetic": {"f64": "FP64", "f32w": "FP64 (FP32 storage)",
-                               "f32c": "float-float compensated (FP32 storage)"}[form]}
+                "arithmetic": "FP64" if form == "f64" else ARITHMETIC[form]}
         if ref is not None:
             L, du, dk, _ = res["r"]
             rel = lambda x, r: float(((x - r).abs().max() / r.abs().max()).item())
@@ -438,6 +447,7 @@ def reduced_forms(w, steps, warmup, ms64, forms=("f32w", "bf16s", "f32c")):
         big = ref_q.abs().max()
         out["bk5_p7_" + form] = {
             "value": round(w["dofs"] / (ms * 1e-3) * 1e-9, 3), "unit": "GDoF/s", "ms_per_step": round(ms, 5),
+            "arithmetic": ARITHMETIC[form],
             "speedup_vs_f64": round(ms64 / ms, 3), "bytes_per_dof": nb,
             "frac": round(nb * w["dofs"] / (ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4),
             "relerr_vs_f64_same_inputs": float(((v.double() - ref_q).abs().max() / big).item()),
@@ -556,6 +566,27 @@ def mixed_layouts(steps, warmup, nelmt=200000):
         alone[p] = round(ms, 4)
     out["bk3_alone_ms_by_p"] = alone
     out["bk3_alone_sum_ms"] = round(sum(alone.values()), 4)
+    return out
+
+
+def compensated_summary(line):
+    """The target 'FP32-compensated form >= 1.6x the FP64 GDoF/s at <= 1e-12
+    relative error' read off the lines of the compensated forms only (f32c);
+    f32w, the FP64 operator on FP32 storage, is shown beside it as the byte
+    lever, not as a compensated form."""
+    out = {"target": ">= 1.6x FP64 GDoF/s at <= 1e-12 max-relative error vs FP64 on the same inputs"}
+    met = True
+    for name in ("bk5_p7_f32c", "bk3_p7_f32c", "bk3_adjoint_p6_f32c"):
+        r = line.get(name)
+        if not r:
+            continue
+        err = r["relerr_vs_f64_same_inputs"]
+        err = max(err.values()) if isinstance(err, dict) else err
+        out[name] = {"speedup_vs_f64": r.get("speedup_vs_f64"), "relerr": err}
+        met = met and r.get("speedup_vs_f64", 0) >= 1.6 and err <= 1e-12
+    out["met"] = met
+    out["byte_lever_fp64_arithmetic"] = {n: line[n]["speedup_vs_f64"] for n in ("bk5_p7_f32w", "bk3_p7_f32w_eo")
+                                         if n in line}
     return out
 
 
@@ -682,6 +713,7 @@ def main():
         **lines,
         "bk3_mixed": mix,
     }
+    line["fp32_compensated_target"] = compensated_summary(line)
     print(json.dumps(line), flush=True)
 
 

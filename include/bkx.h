@@ -266,6 +266,33 @@ int bkx_geo_bk5_f64(int p, size_t nelmt, const double* dbasis, const double* qwe
                     double* G, bkx_stream_t stream);
 
 /* ---------------------------------------------------------------------- */
+/* Layouts of the geometric factors (the reference harness's helpers,      */
+/* include/bk_harness.h:87-102, 207-244, on the device; pure copies, so    */
+/* the results are bit-identical to the host helpers).                      */
+/* ---------------------------------------------------------------------- */
+
+/* BK5, p = 1..7 (nq = p + 1): from the interleaved six factors G6i(e, i, j,
+ * s, k) that bkx_bk5_apply_* read, the factors with F = 6 (rr rs rt ss st tt)
+ * or F = 9 (the full symmetric tensor, row-major: rr rs rt sr ss st tr ts tt)
+ * per point in layout 0 (factor-major G(e, f, i, j, k)), 1 (interleaved
+ * G(e, i, j, f, k)) or 2 (column G(e, f, k, i, j)).  Replaces
+ * bk::make_bk5_G(G6i, nelmt, nq, F, layout) of include/bk_harness.h:217-244.
+ * BKX_ERROR_ARG for another F or layout. */
+int bkx_bk5_make_G_f64(int p, size_t nelmt, const double* G6i, int F, int layout, double* G, bkx_stream_t stream);
+int bkx_bk5_make_G_f32(int p, size_t nelmt, const float* G6i, int F, int layout, float* G, bkx_stream_t stream);
+/* The inverse: factors in (F, layout) as above -> G6i, so that a caller
+ * holding factor-major, column or nine-factor G can apply BK5 (for F = 9 the
+ * entries on and above the diagonal are taken).  No reference counterpart. */
+int bkx_bk5_interleave_G_f64(int p, size_t nelmt, int F, int layout, const double* G, double* G6i,
+                             bkx_stream_t stream);
+int bkx_bk5_interleave_G_f32(int p, size_t nelmt, int F, int layout, const float* G, float* G6i,
+                             bkx_stream_t stream);
+/* BK1, p = 1..8 (nq = p + 2): JxW(e, p, q, r) -> JxW_t(e, r, q, p).  Replaces
+ * bk::transpose_jxw(JxW, nelmt, nq) of include/bk_harness.h:87-102. */
+int bkx_bk1_transpose_jxw_f64(int p, size_t nelmt, const double* JxW, double* JxW_t, bkx_stream_t stream);
+int bkx_bk1_transpose_jxw_f32(int p, size_t nelmt, const float* JxW, float* JxW_t, bkx_stream_t stream);
+
+/* ---------------------------------------------------------------------- */
 /* Element restriction (no reference counterpart: the reference applies     */
 /* its operators to E-vectors only).  offsets[e * elemsize + t] is the     */
 /* global node (0 .. lsize - 1) of element e's local node t, as libCEED's   */
@@ -311,9 +338,14 @@ typedef struct bkx_graph bkx_graph;
 
 /* Builds a plan (host work plus a few small device arrays): the elements
  * sorted by order and cut into tiles of rounds_by_order[p] x epb elements,
- * epb = floor(256 / q^2) / div_by_order[p] (div 1 or 2; either array may be
- * NULL for 1), kept both grouped by order and sorted by decreasing cost.
- * grid <= 0 picks the number of resident workgroups of the device. */
+ * epb = floor(256 / q^2) / div_by_order[p] (div 1 or 2), kept both grouped
+ * by order and sorted by decreasing cost.  div_by_order and rounds_by_order
+ * are indexed by the order: each holds 9 entries, [0] unused and [p] for
+ * p = 1..8 (a shorter array is read out of bounds); either may be NULL for
+ * all 1.  grid <= 0 picks the number of resident workgroups of the device.
+ * The single launch is persistent (workgroup g takes tiles g, g + grid, ...)
+ * when some rounds_by_order[p] > 1, or when grid > 0 and rounds_by_order is
+ * NULL; otherwise it runs one workgroup per tile. */
 int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
                           const int32_t* rounds_by_order, int grid, bkx_mixed_plan** plan);
 /* The same plan for the grouped layout: each order's elements contiguous
@@ -330,8 +362,10 @@ int bkx_mixed_plan_info(const bkx_mixed_plan* plan, size_t* ndof, size_t* ngeo, 
 int bkx_mixed_plan_offsets(const bkx_mixed_plan* plan, int64_t* uoff, int64_t* goff);
 
 /* One launch over every tile: one workgroup per tile (default), or
- * persistent (workgroup g takes tiles g, g + grid, ...) when the plan was
- * made with rounds_by_order or a grid > 0. */
+ * persistent when the plan was made so (see bkx_mixed_plan_create).  The
+ * array sizes cannot be checked here: bases must hold the 708 doubles of the
+ * packed bases (sum over p = 1..8 of (p+1)(p+2) + (p+2)^2), u and v ndof and
+ * G ngeo doubles of bkx_mixed_plan_info. */
 int bkx_bk3_mixed_apply_f64(const bkx_mixed_plan* plan, const double* bases, const double* G, const double* in,
                             double* out, bkx_stream_t stream);
 /* One launch per order, each by a kernel compiled for that order alone. */

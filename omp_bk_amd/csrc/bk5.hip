@@ -939,7 +939,9 @@ int launch_bk5(int variant, size_t nelmt, const double* D, const TS* G, const TS
     case 26:
         return launch_bk5t_v<NQ, 1, TS, false, 4, true, 2, true>(nelmt, D, G, u, v, s);
     case 27:
-        return launch_bk5t_v<NQ, 2, TS, false, 4, true, 4, true>(nelmt, D, G, u, v, s);
+        // four waves per SIMD do not fit the FP64 registers at nq = 8: three there
+        return launch_bk5t_v<NQ, 2, TS, false, (NQ == 8 && sizeof(TS) == 8) ? 3 : 4, true, 4, true>(nelmt, D, G, u,
+                                                                                                     v, s);
     case 28:
         return launch_bk5t_v<NQ, 1, TS, false, 1, true, 8, true>(nelmt, D, G, u, v, s);
     case 29:
@@ -1016,7 +1018,8 @@ int launch_bk5_gather(int variant, size_t nelmt, const double* D, const double* 
     case 18:
         return launch_bk5t_v<NQ, 2, double, false, 1, true, 4, false, true>(nelmt, D, G, uL, v, s, uidx);
     case 27:
-        return launch_bk5t_v<NQ, 2, double, false, 4, true, 4, true, true>(nelmt, D, G, uL, v, s, uidx);
+        return launch_bk5t_v<NQ, 2, double, false, NQ == 8 ? 3 : 4, true, 4, true, true>(nelmt, D, G, uL, v, s,
+                                                                                         uidx);
     case 34:
         return launch_bk5t_v<NQ, 1, double, false, 1, 2, 8, false, true>(nelmt, D, G, uL, v, s, uidx);
     case 35:

@@ -34,11 +34,6 @@ struct Bk5FfShape {
     static constexpr int box = NQ * plane;
 };
 
-__device__ __forceinline__ void pin(ff& x)
-{
-    asm volatile("" : "+v"(x.h), "+v"(x.l));
-}
-
 __device__ __forceinline__ void pin(dot2& x)
 {
     asm volatile("" : "+v"(x.s), "+v"(x.c));

@@ -84,6 +84,15 @@ def zipf_orders(nelmt=200000, s=1.2, seed=2024):
     return order
 
 
+# doubles of the packed bases: B_p (n x q) and D_p (q x q) for p = 1..8
+PACKED_BASES = sum((p + 1) * (p + 2) + (p + 2) ** 2 for p in range(1, 9))
+
+
+def prepared_size():
+    """Doubles of the even-odd forms of the packed bases."""
+    return int(_bind().bkx_bk3_mixed_prepared_size())
+
+
 def packed_bases():
     """B_p (n x q) then D_p (q x q) for p = 1..8 (the layout of include/bkx.h)."""
     parts = []
@@ -164,6 +173,9 @@ class MixedPlan:
         self.nelmt = self.order.size
         div = None if div_by_order is None else np.ascontiguousarray(div_by_order, dtype=np.int32)
         rnd = None if rounds_by_order is None else np.ascontiguousarray(rounds_by_order, dtype=np.int32)
+        for name, a in (("div_by_order", div), ("rounds_by_order", rnd)):
+            if a is not None and a.shape != (9,):
+                raise ValueError("%s is indexed by the order: 9 entries, [0] unused and [p] for p = 1..8" % name)
         self._div, self._rnd = div, rnd
         h = _vp()
         name = "bkx_mixed_plan_create_grouped" if grouped else "bkx_mixed_plan_create"
@@ -193,27 +205,33 @@ class MixedPlan:
         except Exception:
             pass
 
-    def _args(self, bases, G, u, v):
-        # bases: the packed bases, or their even-odd forms (prepared_bases)
+    def _args(self, bases, G, u, v, prepared=False):
+        # bases: the packed bases, or with prepared their even-odd forms
+        # (prepared_bases); the kernels read every order's entries, so a
+        # short array would be read out of bounds on the device
         for name, t in (("bases", bases), ("G", G), ("u", u), ("v", v)):
             if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64 and t.is_contiguous()):
                 raise TypeError("%s must be a contiguous float64 tensor on a HIP device" % name)
         if u.numel() != self.ndof or v.numel() != self.ndof or G.numel() != self.ngeo:
             raise ValueError("u and v must hold %d values and G %d" % (self.ndof, self.ngeo))
+        nb = prepared_size() if prepared else PACKED_BASES
+        if bases.numel() != nb:
+            raise ValueError("bases must hold %d doubles (%s), got %d"
+                             % (nb, "even-odd forms" if prepared else "packed B and D of p = 1..8", bases.numel()))
         return bases.data_ptr(), G.data_ptr(), u.data_ptr(), v.data_ptr()
 
     def apply(self, bases, G, u, v, prepared=False):
         """One launch over all tiles, on the current torch stream; with
         prepared, bases are the even-odd forms of prepared_bases."""
         name = "bkx_bk3_mixed_apply_prepared_f64" if prepared else "bkx_bk3_mixed_apply_f64"
-        _check(name, getattr(_bind(), name)(self.handle, *self._args(bases, G, u, v),
+        _check(name, getattr(_bind(), name)(self.handle, *self._args(bases, G, u, v, prepared),
                                             torch.cuda.current_stream().cuda_stream))
         return v
 
     def apply_by_order(self, bases, G, u, v, prepared=False):
         """One launch per order."""
         name = "bkx_bk3_mixed_apply_by_order_prepared_f64" if prepared else "bkx_bk3_mixed_apply_by_order_f64"
-        _check(name, getattr(_bind(), name)(self.handle, *self._args(bases, G, u, v),
+        _check(name, getattr(_bind(), name)(self.handle, *self._args(bases, G, u, v, prepared),
                                             torch.cuda.current_stream().cuda_stream))
         return v
 
@@ -231,7 +249,7 @@ class MixedGraph:
     def __init__(self, plan, bases, G, u, v, by_order, prepared=False):
         h = _vp()
         name = "bkx_bk3_mixed_graph_create_prepared" if prepared else "bkx_bk3_mixed_graph_create"
-        _check(name, getattr(_bind(), name)(plan.handle, *plan._args(bases, G, u, v), int(by_order),
+        _check(name, getattr(_bind(), name)(plan.handle, *plan._args(bases, G, u, v, prepared), int(by_order),
                                             ctypes.byref(h)))
         self.handle = h
         self._keep = (plan, bases, G, u, v)

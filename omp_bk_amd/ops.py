@@ -363,3 +363,62 @@ def geo_bk5(p, dbasis, qweights, coords, out=None):
         _lib.call("bkx_geo_bk5_f64", p, ne, dbasis.data_ptr(), qweights.data_ptr(), coords.data_ptr(),
                   out.data_ptr(), _stream(coords))
     return out
+
+
+_LAYOUT_SUFFIX = {torch.float64: "f64", torch.float32: "f32"}
+
+
+def bk5_make_G(p, G6i, F=6, layout=0, out=None):
+    """BK5's interleaved six factors G6i (E, n, n, 6, n) in another layout
+    (make_bk5_G, include/bk_harness.h:217-244): F = 6 or 9 factors per point,
+    layout 0 factor-major (E, F, n, n, n), 1 interleaved (E, n, n, F, n), 2
+    column (E, F, n, n, n) indexed (f, k, i, j).  FP64 or FP32, as G6i."""
+    _order_ok(5, p)
+    n = p + 1
+    if G6i.dtype not in _LAYOUT_SUFFIX or G6i.numel() % (6 * n ** 3):
+        raise ValueError("G6i must be FP64 or FP32 and hold whole elements of 6 x %d factors" % n ** 3)
+    ne = G6i.numel() // (6 * n ** 3)
+    _check("G6i", G6i, ne * 6 * n ** 3, G6i.dtype)
+    if out is None:
+        out = torch.empty(ne * F * n ** 3, dtype=G6i.dtype, device=G6i.device)
+    _check("out", out, ne * F * n ** 3, G6i.dtype)
+    with _on(G6i):
+        _lib.call("bkx_bk5_make_G_" + _LAYOUT_SUFFIX[G6i.dtype], p, ne, G6i.data_ptr(), F, layout, out.data_ptr(),
+                  _stream(G6i))
+    return out
+
+
+def bk5_interleave_G(p, G, F=6, layout=0, out=None):
+    """The inverse of bk5_make_G: F = 6 or 9 factors in the given layout ->
+    the interleaved six factors that bk5 reads."""
+    _order_ok(5, p)
+    n = p + 1
+    if G.dtype not in _LAYOUT_SUFFIX or G.numel() % (F * n ** 3):
+        raise ValueError("G must be FP64 or FP32 and hold whole elements of %d x %d factors" % (F, n ** 3))
+    ne = G.numel() // (F * n ** 3)
+    _check("G", G, ne * F * n ** 3, G.dtype)
+    if out is None:
+        out = torch.empty(ne * 6 * n ** 3, dtype=G.dtype, device=G.device)
+    _check("out", out, ne * 6 * n ** 3, G.dtype)
+    with _on(G):
+        _lib.call("bkx_bk5_interleave_G_" + _LAYOUT_SUFFIX[G.dtype], p, ne, F, layout, G.data_ptr(), out.data_ptr(),
+                  _stream(G))
+    return out
+
+
+def bk1_transpose_jxw(p, JxW, out=None):
+    """BK1's JxW(e, p, q, r) -> JxW_t(e, r, q, p) (transpose_jxw,
+    include/bk_harness.h:87-102).  FP64 or FP32, as JxW."""
+    _order_ok(1, p)
+    nq = p + 2
+    if JxW.dtype not in _LAYOUT_SUFFIX or JxW.numel() % nq ** 3:
+        raise ValueError("JxW must be FP64 or FP32 and hold whole elements of %d values" % nq ** 3)
+    ne = JxW.numel() // nq ** 3
+    _check("JxW", JxW, ne * nq ** 3, JxW.dtype)
+    if out is None:
+        out = torch.empty_like(JxW)
+    _check("out", out, ne * nq ** 3, JxW.dtype)
+    with _on(JxW):
+        _lib.call("bkx_bk1_transpose_jxw_" + _LAYOUT_SUFFIX[JxW.dtype], p, ne, JxW.data_ptr(), out.data_ptr(),
+                  _stream(JxW))
+    return out

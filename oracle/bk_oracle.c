@@ -17,7 +17,9 @@
  *     whose divergence sums interleave the three directions in one loop;
  *   - the float-float error-free transformations (openmp/float_float.h:73-134)
  *     and the float-float BK1 kernel (openmp/BK1_ff.cpp:73-241);
- *   - the two error metrics (bk_harness.h:63-83, BK1_ff.cpp:751-763).
+ *   - the two error metrics (bk_harness.h:63-83, BK1_ff.cpp:751-763);
+ *   - the layout helpers make_bk5_G and transpose_jxw (bk_harness.h:87-102,
+ *     207-244).
  * It adds what the reference does not have: the BK3 operator with a
  * per-quadrature-point coefficient kappa scaling the six factors and the
  * adjoint of L = 1/2 |A(kappa) u - b|^2 (gradients in u and kappa).
@@ -747,6 +749,54 @@ void bko_bk1_ff_sumfact(int nq, size_t nelmt, const double* basis, const double*
         free(Bl);
         free(Bh);
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Layouts of the geometric factors (include/bk_harness.h)                   */
+/* ------------------------------------------------------------------------ */
+
+/* The full-tensor entry (a, b), row-major index 3a + b, of the six symmetric
+ * factors rr rs rt ss st tt (bk_harness.h:215). */
+static const int bko_sym9[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+
+/* Index of point (i, j, k), factor f of F, within one element's factors in
+ * layout 0 (factor-major G(f, i, j, k)), 1 (interleaved G(i, j, f, k)) or 2
+ * (column G(f, k, i, j)). */
+static size_t bko_g_index(int nq, int F, int layout, int i, int j, int k, int f)
+{
+    if (layout == 1) return ((size_t)(i * nq + j) * F + f) * nq + k;
+    if (layout == 2) return ((size_t)(f * nq + k) * nq + i) * nq + j;
+    return ((size_t)(f * nq + i) * nq + j) * nq + k;
+}
+
+/* make_bk5_G (bk_harness.h:217-244): from the interleaved six factors
+ * G6i(e, i, j, s, k), the factors with F = 6 (symmetric) or 9 (the full
+ * tensor, row-major) per point in the given layout.  Pure data movement. */
+void bko_make_bk5_G(int nq, size_t nelmt, const double* G6i, int F, int layout, double* G)
+{
+    const size_t nq3 = (size_t)nq * nq * nq;
+    for (size_t e = 0; e < nelmt; ++e) {
+        const double* src = G6i + e * 6 * nq3;
+        double* dst = G + e * (size_t)F * nq3;
+        for (int f = 0; f < F; ++f) {
+            const int s = F == 6 ? f : bko_sym9[f];
+            for (int i = 0; i < nq; ++i)
+                for (int j = 0; j < nq; ++j)
+                    for (int k = 0; k < nq; ++k)
+                        dst[bko_g_index(nq, F, layout, i, j, k, f)] = src[bko_g_index(nq, 6, 1, i, j, k, s)];
+        }
+    }
+}
+
+/* transpose_jxw (bk_harness.h:87-102): JxW(e, p, q, r) -> JxW_t(e, r, q, p). */
+void bko_transpose_jxw(int nq, size_t nelmt, const double* JxW, double* JxW_t)
+{
+    const size_t nq3 = (size_t)nq * nq * nq;
+    for (size_t e = 0; e < nelmt; ++e)
+        for (int r = 0; r < nq; ++r)
+            for (int q = 0; q < nq; ++q)
+                for (int p = 0; p < nq; ++p)
+                    JxW_t[e * nq3 + W3(nq, r, q, p)] = JxW[e * nq3 + W3(nq, p, q, r)];
 }
 
 /* ------------------------------------------------------------------------ */

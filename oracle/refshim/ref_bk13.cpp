@@ -86,6 +86,16 @@ void bkref_make_bk5_G(const double* G6i, std::size_t nelmt, int nq, int F, int l
     }
 }
 
+// BK1's transposed JxW (bk_harness.h:87-102); float-representable values
+void bkref_transpose_jxw(const double* JxW, std::size_t nelmt, int nq, double* JxW_t)
+{
+    const std::size_t nq3 = std::size_t(nq) * nq * nq;
+    const std::vector<float> t = bk::transpose_jxw(to_float(JxW, nelmt * nq3), nelmt, nq);
+    for (std::size_t x = 0; x < t.size(); ++x) {
+        JxW_t[x] = t[x];
+    }
+}
+
 // Test data (bk_common.h:63-107)
 void bkref_make_test_basis_f64(int rows, int cols, double* m)
 {

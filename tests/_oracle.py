@@ -192,3 +192,36 @@ def ref_bk1_ff(variant, p, basis, jxw, u):
                             _ptr(_arr(jxw)), _ptr(u), _ptr(out))
     assert rc == 0
     return out
+
+
+def make_bk5_G(nq, G6i, F, layout):
+    """make_bk5_G of the oracle: G6i(e, i, j, s, k) -> F factors in layout
+    0 (factor-major), 1 (interleaved) or 2 (column)."""
+    G6i = _arr(G6i)
+    ne = G6i.size // (6 * nq ** 3)
+    out = np.empty(ne * F * nq ** 3)
+    oracle().bko_make_bk5_G(ctypes.c_int(nq), _sz(ne), _ptr(G6i), ctypes.c_int(F), ctypes.c_int(layout), _ptr(out))
+    return out
+
+
+def transpose_jxw(nq, jxw):
+    jxw = _arr(jxw)
+    out = np.empty_like(jxw)
+    oracle().bko_transpose_jxw(ctypes.c_int(nq), _sz(jxw.size // nq ** 3), _ptr(jxw), _ptr(out))
+    return out
+
+
+def ref_make_bk5_G(nq, G6i, F, layout):
+    """The reference's make_bk5_G (float inputs: G6i must be float-exact)."""
+    G6i = _arr(G6i)
+    ne = G6i.size // (6 * nq ** 3)
+    out = np.empty(ne * F * nq ** 3)
+    ref().bkref_make_bk5_G(_ptr(G6i), _sz(ne), ctypes.c_int(nq), ctypes.c_int(F), ctypes.c_int(layout), _ptr(out))
+    return out
+
+
+def ref_transpose_jxw(nq, jxw):
+    jxw = _arr(jxw)
+    out = np.empty_like(jxw)
+    ref().bkref_transpose_jxw(_ptr(jxw), _sz(jxw.size // nq ** 3), ctypes.c_int(nq), _ptr(out))
+    return out

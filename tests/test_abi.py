@@ -78,3 +78,46 @@ def test_python_errors():
     with pytest.raises(_lib.BkxError) as ei:
         _lib.call("bkx_bk5_apply_f64", 0, 1, None, None, None, None, None)
     assert ei.value.status == 1
+
+
+def test_drivers_reject_bad_arguments_without_device():
+    """The precision drivers and the C++ shim caller check the order before
+    any device work (the reference prints "unsupported polynomial order"
+    and exits 1)."""
+    from omp_bk_amd import study
+    for kind, p in ((1, 9), (3, 0), (5, 8)):
+        r = subprocess.run([os.path.join(study.BIN, "BK%d_ff" % kind), str(p), "10", "1"], capture_output=True,
+                           text=True, timeout=60)
+        assert r.returncode == 1 and "unsupported polynomial order" in r.stderr, r.stderr
+    r = subprocess.run([os.path.join(study.BIN, "bk_shim_check"), "5", "8", "1", "/dev/null", "/dev/null"],
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "unsupported" in r.stderr, r.stderr
+    r = subprocess.run([os.path.join(study.BIN, "bk_shim_check")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "usage" in r.stderr
+
+
+def test_layout_helpers_check_arguments_without_device():
+    L = _lib.lib()
+    assert L.bkx_bk5_make_G_f64(8, 10, None, 6, 0, None, None) == 1
+    assert L.bkx_bk5_make_G_f64(7, 10, None, 7, 0, None, None) == 5
+    assert L.bkx_bk5_make_G_f64(7, 10, None, 9, 3, None, None) == 5
+    assert L.bkx_bk5_make_G_f64(7, 0, None, 9, 2, None, None) == 0
+    assert L.bkx_bk5_interleave_G_f32(7, 10, 9, 2, None, None, None) == 2
+    assert L.bkx_bk1_transpose_jxw_f64(9, 10, None, None, None) == 1
+    assert L.bkx_bk1_transpose_jxw_f32(8, 10, None, None, None) == 2
+    # FP32-compensated adjoint entry points: order and pointers checked first
+    assert L.bkx_bk3_adjoint_f32c(9, 10, *([None] * 10)) == 1
+    assert L.bkx_bk3_adjoint_f32c(6, 10, *([None] * 10)) == 2
+    assert L.bkx_bk3_vjp_f32c(6, 0, *([None] * 9)) == 0
+    assert L.bkx_bk3_apply_kappa_f32c(6, 10, *([None] * 7)) == 2
+
+
+def test_mixed_plan_checks_per_order_arrays():
+    """div_by_order and rounds_by_order are indexed by the order (9 entries):
+    a shorter array is refused on the host before the C ABI reads it."""
+    from omp_bk_amd import mixed
+    order = np.array([1, 2, 3], dtype=np.int32)
+    with pytest.raises(ValueError):
+        mixed.MixedPlan(order, div_by_order=[1] * 8)
+    with pytest.raises(ValueError):
+        mixed.MixedPlan(order, rounds_by_order=[1] * 10)

@@ -112,3 +112,78 @@ def test_study_pick(tmp_path):
     assert "v3" not in {r["variant"] for r in rows}
     best, bad = study.best_by_median(rows)
     assert not bad and ("hip", 2) in best
+
+
+def _run_rc(args, env=None):
+    e = dict(os.environ)
+    e.update(env or {})
+    print("run:", " ".join(args), env or "", flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, env=e, timeout=120)
+    return r.returncode, r.stdout + r.stderr
+
+
+def _ff_errors(text):
+    """The gated errors of the ok/FAIL line, by form name."""
+    line = [ln for ln in text.splitlines() if ln.startswith(("ok:", "FAIL:"))]
+    assert len(line) == 1, text
+    body = line[0].split(":", 1)[1].rsplit("(tol", 1)[0]
+    out = {}
+    for part in body.split(","):
+        name, val = part.split(" rel. error vs double = ")
+        out[name.strip()] = float(val)
+    return line[0].split(":")[0], out
+
+
+@pytest.mark.parametrize("kind,p", [(1, 1), (1, 3), (1, 8), (3, 2), (3, 5), (3, 8), (5, 3), (5, 7)])
+def test_precision_driver_ok(kind, p):
+    """bin/BKn_ff (openmp/BK1_ff.cpp:767-859): every form on the same LCG
+    data, the float-float forms within the reference's default BK_TOL of
+    1e-10 of the FP64 apply (in fact ~1e-14), exit status 0; the FP64 row's
+    norm equals the oracle's."""
+    nelmt = 1000
+    rc, out = _run_rc([os.path.join(BIN, "BK%d_ff" % kind), str(p), str(nelmt), "2"], {"BK_RANDOM": "1"})
+    assert rc == 0, out
+    status, errs = _ff_errors(out)
+    assert status == "ok"
+    expect = {"f32c", "f32w"} | ({"float-float"} if kind != 5 else set())
+    assert set(errs) == expect, errs
+    assert all(e <= 1e-12 for e in errs.values()), errs
+    nm = p + 1
+    nq = p + 1 if kind == 5 else p + 2
+    u, Q = O.fill_random(nelmt * nm ** 3, nelmt * (1 if kind == 1 else 6) * nq ** 3)
+    ref = np.linalg.norm(_oracle_out(kind, p, u, Q))
+    row = [ln for ln in out.splitlines() if ln.startswith("double")][0]
+    assert abs(float(row.split("norm =")[1]) - ref) <= 1e-9 * ref
+    if kind == 5 and p == 7:
+        assert "bf16s" in out
+
+
+def test_precision_driver_fail_exit_status():
+    """A tolerance below the float-float error level makes the driver print
+    FAIL and exit 1, as the reference's does."""
+    rc, out = _run_rc([os.path.join(BIN, "BK1_ff"), "3", "100", "1"], {"BK_RANDOM": "1", "BK_TOL": "1e-300"})
+    assert rc == 1, out
+    status, _ = _ff_errors(out)
+    assert status == "FAIL"
+
+
+@pytest.mark.parametrize("kind,p", [(1, p) for p in range(1, 9)] + [(3, p) for p in range(1, 9)]
+                         + [(5, p) for p in range(1, 8)])
+def test_cpp_shim_caller_golden(kind, p, tmp_path):
+    """The C++ drop-in (include/bkx_sumfact.hpp) held in the reference's own
+    kernel pointer types (sumfact_kernel<double>, openmp/bk1_sumfact.h:
+    162-163; BK3's kernel_fn, openmp/offload/bk3_offload.cpp:226) and called
+    synchronously on hipMalloc'd golden inputs (bin/bk_shim_check): the
+    output against the reference's SumFactorization<double, nq>."""
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bk%d_p%d.npz" % (kind, p)))
+    names = {1: ("basis", "JxW", "u"), 3: ("basis", "dbasis", "G", "u"), 5: ("dbasis", "G", "u")}[kind]
+    n = p + 1
+    nelmt = g["u"].size // n ** 3
+    src, dst = tmp_path / "in.bin", tmp_path / "out.bin"
+    np.concatenate([np.ascontiguousarray(g[k], dtype=np.float64).reshape(-1) for k in names]).tofile(src)
+    rc, out = _run_rc([os.path.join(BIN, "bk_shim_check"), str(kind), str(p), str(nelmt), str(src), str(dst)])
+    assert rc == 0, out
+    assert "SumFactorization<double, %d>" % (p + 1 if kind == 5 else p + 2) in out
+    got = np.fromfile(dst, dtype=np.float64)
+    assert got.size == g["sumfact"].size
+    assert O.rel_maxdiff(got, g["sumfact"]) <= 1e-13

@@ -213,3 +213,25 @@ def test_grouped_layout_same_bits():
     graph.close()
     persist.close()
     gp.close()
+
+
+def test_bases_size_checked():
+    """The packed bases (708 doubles) and their even-odd forms (756) are
+    checked against the form asked for: a short or mismatched array raises
+    instead of being read out of bounds on the device."""
+    order = np.array([1, 3, 5, 8], dtype=np.int32)
+    plan = mixed.MixedPlan(order)
+    B = torch.as_tensor(mixed.packed_bases(), device=DEV)
+    P = mixed.prepared_bases(B)
+    G = torch.zeros(plan.ngeo, dtype=torch.float64, device=DEV)
+    u = torch.zeros(plan.ndof, dtype=torch.float64, device=DEV)
+    v = torch.empty_like(u)
+    with pytest.raises(ValueError):
+        plan.apply(B, G, u, v, prepared=True)
+    with pytest.raises(ValueError):
+        plan.apply(P, G, u, v)
+    with pytest.raises(ValueError):
+        plan.apply_by_order(B[:-1], G, u, v)
+    plan.apply(B, G, u, v)
+    plan.apply(P, G, u, v, prepared=True)
+    torch.cuda.synchronize()

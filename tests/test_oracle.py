@@ -267,3 +267,34 @@ def test_adjoint_finite_differences(p):
         Lm = O.bk3_adjoint(nq, B, D, G, kappa - h * dk, u, b)[0]
         fd = (Lp - Lm) / (2 * h)
         assert abs(fd - dldk @ dk) <= 1e-6 * (abs(fd) + 1e-30)
+
+
+@pytest.mark.parametrize("nq", range(2, 9))
+def test_layout_helpers(nq):
+    """make_bk5_G and transpose_jxw of the oracle: bit for bit against the
+    reference build (bk_harness.h:87-102, 217-244), and pinned without it by
+    what the layouts fix: layout 1 with six factors is the identity, the
+    nine-factor tensor is symmetric with the six factors on and above its
+    diagonal, layouts 0 and 2 are the numpy transpositions of layout 1, and
+    transposing JxW twice is the identity."""
+    E = 3
+    G6i, jxw = O.fill_random(E * 6 * nq ** 3, E * nq ** 3, seed=nq)
+    jxw = jxw.astype(np.float32).astype(np.float64)      # the reference helpers take floats
+    g6 = G6i.reshape(E, nq, nq, 6, nq)                                  # (e, i, j, s, k)
+    assert np.array_equal(O.make_bk5_G(nq, G6i, 6, 1), G6i)
+    assert np.array_equal(O.make_bk5_G(nq, G6i, 6, 0).reshape(E, 6, nq, nq, nq), g6.transpose(0, 3, 1, 2, 4))
+    assert np.array_equal(O.make_bk5_G(nq, G6i, 6, 2).reshape(E, 6, nq, nq, nq), g6.transpose(0, 3, 4, 1, 2))
+    g9 = O.make_bk5_G(nq, G6i, 9, 0).reshape(E, 3, 3, nq, nq, nq)       # (e, a, b, i, j, k)
+    assert np.array_equal(g9, g9.transpose(0, 2, 1, 3, 4, 5))
+    upper = [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+    for s, (a, b) in enumerate(upper):
+        assert np.array_equal(g9[:, a, b], g6[:, :, :, s, :])
+    t = O.transpose_jxw(nq, jxw)
+    assert np.array_equal(t.reshape(E, nq, nq, nq), jxw.reshape(E, nq, nq, nq).transpose(0, 3, 2, 1))
+    assert np.array_equal(O.transpose_jxw(nq, t), jxw)
+    if O.ref() is None:
+        pytest.skip("reference build (oracle/_ref) not present")
+    for F in (6, 9):
+        for layout in (0, 1, 2):
+            assert np.array_equal(O.make_bk5_G(nq, G6i, F, layout), O.ref_make_bk5_G(nq, G6i, F, layout)), (F, layout)
+    assert np.array_equal(t, O.ref_transpose_jxw(nq, jxw))
