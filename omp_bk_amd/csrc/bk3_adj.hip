@@ -15,9 +15,17 @@
 // gradient of u, F = G P u kept in registers, the divergence of kappa F
 // (loss mode: r = A u - b and 1/2 |r|^2 per element), the gradient of r (or
 // w), dL/dkappa = P r . F, and the divergence of kappa G P r.  G and kappa
-// are read twice, the second time while the element's first read is still
-// in the L2 cache.  The phases are those of the transposing BK3 mapping
-// (bk3_phases.hpp).
+// are read twice.  At about 2.3 resident waves per SIMD some 300 elements
+// per XCD are in flight and their factors (28.7 KB per element at p = 6 in
+// FP64) overflow the 4 MB L2 between the two reads, so the second read is
+// largely served from the MALL (rocprofv3: 1.85x the algorithmic read
+// bytes).  Keeping the first read's factors in registers (56 values per
+// lane) needs 256 VGPRs and one wave per SIMD, which measured 10-17%
+// slower; what is kept is that every other stream -- u, b (w), the
+// outputs and the second, last read of G and kappa -- is non-temporal, so
+// that the first read's lines are the ones the L2 holds (p = 6, E = 16384:
+// 169 -> 156 us in FP64, 128 -> 119 us with FP32 storage, same bits).  The
+// phases are those of the transposing BK3 mapping (bk3_phases.hpp).
 //
 // TS is the storage type of the inputs u, b (or w), kappa and G: double for
 // the FP64 entry points, float for the FP32-storage forms, which compute
@@ -159,7 +167,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
         TS raw[NM];
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            raw[i] = base[ee * E::nm3 + i * E::nm2 + lm];
+            raw[i] = ld_stream(base + ee * E::nm3 + i * E::nm2 + lm);
         }
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
@@ -220,7 +228,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
             if (active && modal) {
 #pragma unroll
                 for (int i = 0; i < NM; ++i) {
-                    out_u[e * E::nm3 + i * E::nm2 + l] = as_f64(o[i]);
+                    st_stream(out_u + e * E::nm3 + i * E::nm2 + l, as_f64(o[i]));
                 }
             }
             return;
@@ -278,9 +286,9 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
             TS graw[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                graw[f] = ge[f * E::nq3 + p * nq2];
+                graw[f] = ld_stream(ge + f * E::nq3 + p * nq2);   // the last use of G and kappa
             }
-            const TS kraw = ke[p * nq2];
+            const TS kraw = ld_stream(ke + p * nq2);
             L g[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
@@ -291,7 +299,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
             const V qs = el.B[xx];
             const V qt = el.C[xx];
             if (active) {
-                out_k[e * E::nq3 + p * nq2 + l] = as_f64(dot3(qrr[p], F0[p], qs, F1[p], qt, F2[p]));
+                st_stream(out_k + e * E::nq3 + p * nq2 + l, as_f64(dot3(qrr[p], F0[p], qs, F1[p], qt, F2[p])));
             }
             rr[p] = scale(kap, flux(qrr[p], qs, qt, g[0], g[1], g[2]));
             el.B[xx] = scale(kap, flux(qrr[p], qs, qt, g[1], g[3], g[4]));
@@ -304,7 +312,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
     if (active && modal) {
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            out_u[e * E::nm3 + i * E::nm2 + l] = as_f64(o2[i]);
+            st_stream(out_u + e * E::nm3 + i * E::nm2 + l, as_f64(o2[i]));
         }
     }
 }
