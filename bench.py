@@ -32,6 +32,8 @@ ARITHMETIC = {
     "f32w_eo": "FP64 arithmetic on FP32 storage, even-odd bases (not compensated)",
     "f32c": "FP32-compensated: float-float (Dot2) arithmetic on FP32 storage",
     "bf16s": "split-BF16 MFMA products accumulated in FP32 (reduced precision)",
+    "oz": "FP32-compensated on the matrix cores: exact 8-bit digit products on BF16 MFMA, exact FP32 level sums, "
+          "FP64 level recombination and pointwise flux",
 }
 
 
@@ -41,7 +43,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk5_bf16s", "bk3", "copy", "mixed",
+    ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk5_bf16s", "bk5_oz", "bk3", "copy", "mixed",
                                        "adjoint_f64", "adjoint_f32w", "adjoint_f32c", "adjoint_f64_eo",
                                        "adjoint_f32w_eo"],
                     default=None,
@@ -575,16 +577,18 @@ def compensated_summary(line):
     f32w, the FP64 operator on FP32 storage, is shown beside it as the byte
     lever, not as a compensated form."""
     out = {"target": ">= 1.6x FP64 GDoF/s at <= 1e-12 max-relative error vs FP64 on the same inputs"}
-    met = True
-    for name in ("bk5_p7_f32c", "bk3_p7_f32c", "bk3_adjoint_p6_f32c"):
+    met = []
+    for name in ("bk5_p7_oz", "bk5_p7_f32c", "bk3_p7_f32c", "bk3_adjoint_p6_f32c"):
         r = line.get(name)
         if not r:
             continue
         err = r["relerr_vs_f64_same_inputs"]
         err = max(err.values()) if isinstance(err, dict) else err
-        out[name] = {"speedup_vs_f64": r.get("speedup_vs_f64"), "relerr": err}
-        met = met and r.get("speedup_vs_f64", 0) >= 1.6 and err <= 1e-12
-    out["met"] = met
+        ok = r.get("speedup_vs_f64", 0) >= 1.6 and err <= 1e-12
+        out[name] = {"speedup_vs_f64": r.get("speedup_vs_f64"), "relerr": err, "meets": ok}
+        if ok:
+            met.append(name)
+    out["met_by"] = met
     out["byte_lever_fp64_arithmetic"] = {n: line[n]["speedup_vs_f64"] for n in ("bk5_p7_f32w", "bk3_p7_f32w_eo")
                                          if n in line}
     return out
@@ -658,7 +662,7 @@ def main():
     # with, the memory-bound lines next, and the arithmetic-bound
     # float-float form last: timed after it, memory-bound kernels ran up to
     # 15% slower on MI355X.
-    forms = reduced_forms(w5, a.steps, a.warmup, ms5, forms=("f32w", "bf16s"))
+    forms = reduced_forms(w5, a.steps, a.warmup, ms5, forms=("f32w", "bf16s", "oz"))
 
     lines = {}
     for name, make in (("bk3_p7_f64", lambda: bk3_workload(7, 65536)), ("bk3_p5_f64", lambda: bk3_workload(5, 32768)),

@@ -21,6 +21,7 @@ _SIGNATURES = {
     "bkx_bk5_apply_f32c": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk5_apply_f32w": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk5_apply_bf16s": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
+    "bkx_bk5_apply_oz": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk1_apply_ff": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk1_apply_f32c": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],
     "bkx_bk1_apply_f32w": [_c_int, _c_size, _vp, _vp, _vp, _vp, _vp],

@@ -1,0 +1,406 @@
+// bk5_oz.hip -- BK5 collocated Laplacian at p = 7 with FP32 storage and
+// its contractions carried by BF16 MFMA products of exact integer digits
+// (an Ozaki-style error-free splitting; no reference counterpart: the
+// reference runs BK5 in float or double, openmp/BK5.cpp:18-103).
+//
+// Every contraction out(i, x) = sum_n M(i, n) w(n, x) (M = D for the
+// gradient, D^T for the divergence; x runs over the 64 points of the cross
+// plane) is evaluated from digits:
+//   M(i, n) = 2^{e_M(i)} sum_b dM_b(i, n) 2^{-8(b+1)}   (per-row exponent)
+//   w(n, x) = 2^{e_w}    sum_a dw_a(n, x) 2^{-8(a+1)}   (one exponent per
+//                                                        element and field)
+// with six digits each, dM_b and dw_a integers of at most 8 bits (|d| <= 256,
+// exactly representable in BF16), found by the exact chain
+// d = rint(r * 256), r <- r * 256 - d.  The products of digits of one level
+// t = a + b,
+//   C_t(i, x) = sum_{a + b = t} sum_n dM_b(i, n) dw_a(n, x),
+// are integers below 2^20 (at most six pairs of eight products of two
+// 8-bit digits), so their FP32 accumulation in the MFMA is exact in any
+// order.  The output is recombined once, in FP64, from levels 0..5:
+//   out(i, x) = 2^{e_M(i) + e_w - 16} sum_t C_t(i, x) 2^{-8t},
+// whose truncation (digit pairs of level >= 6, digits beyond the sixth)
+// is below 2^-44 of the operands' scale.  Measured against the FP64
+// operator on the same FP32 inputs: ~2e-14 max-relative.
+//
+// MFMA mapping (v_mfma_f32_32x32x8bf16_1k, one wavefront per element): rows
+// of A and C are (level group g, output i), 4 x 8; columns of B and C are 32
+// cross points x; K = 8 is the contraction index n.  Lane l holds
+// A[l % 32][4 (l / 32) + 0..3], B[4 (l / 32) + 0..3][l % 32] and
+// C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32], r = 0..15, so that every lane
+// holds all level groups of its four outputs: the recombination is local to
+// the lane.  Per digit a of w, one MFMA adds A_a((g, i), n) = dM_{g - a}(i, n)
+// into the accumulator of levels 0..3 (a = 0..3), and one adds
+// A'_a((g, i), n) = dM_{4 + g - a}(i, n) (g = 0, 1) into that of levels 4, 5
+// (a = 0..5): ten MFMAs per direction and 32 cross points.  The A operands
+// (digits of D and D^T) are built once per wavefront.
+//
+// Flow per element: u loaded in each direction's layout (four values per
+// lane), split into digits, gradient MFMAs, recombined gradients (FP64) to
+// LDS; per direction of the divergence, the flux field it contracts
+// (rr = g0 qr + g1 qs + g2 qt for i, rs for j, rt for k) at that
+// direction's points, in FP64 from the gradients in LDS and G (FP32,
+// global), split into digits, divergence MFMAs, recombined and summed into
+// v in LDS; v stored in FP64.  FP64 is used for the recombination of the
+// levels and for the pointwise fluxes (9 FMAs per point); every
+// contraction product is an exact BF16 product.
+//
+// Bytes per DoF: u 4 + G 24 (FP32) + v 8 (FP64) = 36, as the other FP32-
+// storage forms.
+#include <cmath>
+
+#include "bkx_device.hpp"
+#include "bkx_launch.hpp"
+
+namespace bkx {
+namespace {
+
+using bf16x4 = short __attribute__((ext_vector_type(4)));
+using fx16 = float __attribute__((ext_vector_type(16)));
+using bf16x2 = __bf16 __attribute__((ext_vector_type(2)));
+using fx2 = float __attribute__((ext_vector_type(2)));
+
+constexpr int kN = 8;          // points per direction (p = 7)
+constexpr int kN3 = 512;
+constexpr int kDig = 6;        // digits per operand
+constexpr int kLoLevels = 4;   // levels 0..3 in the first accumulator
+constexpr int kOzGrid = 4096;  // workgroups (one wavefront each), looping over elements
+
+// frexp exponent: |x| / 2^e in [0.5, 1) (0 for x = 0)
+__device__ __forceinline__ int exp_of(float m)
+{
+    int e = 0;
+    (void)frexpf(m, &e);
+    return e;
+}
+__device__ __forceinline__ int exp_of(double m)
+{
+    int e = 0;
+    (void)frexp(m, &e);
+    return e;
+}
+
+// the digits of X, |X| < 1: X = sum_a d[a] 2^{-8(a+1)} + (a residual below
+// 2^{-49}); every step is exact (the product by 256 and the difference of
+// a value and its nearest integer are representable)
+template <typename T>
+__device__ __forceinline__ void digits(T X, T (&d)[kDig])
+{
+#pragma clang fp contract(off)
+    T r = X;
+#pragma unroll
+    for (int a = 0; a < kDig; ++a) {
+        const T t = r * T(256);
+        const T di = rint(t);
+        d[a] = di;
+        r = t - di;
+    }
+}
+
+// The same digits of an FP64 X as floats: the first three by the FP64
+// chain, the last three by the FP32 chain from the residual rounded to FP32
+// (|r| <= 1/2 after three digits, so the rounding costs at most 2^-25 |r|,
+// 2^-50 of X's scale, below the truncation of the sixth digit).
+__device__ __forceinline__ void digits_mixed(double X, float (&d)[kDig])
+{
+#pragma clang fp contract(off)
+    double r = X;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const double t = r * 256.0;
+        const double di = rint(t);
+        d[a] = float(di);
+        r = t - di;
+    }
+    float rf = float(r);
+#pragma unroll
+    for (int a = 3; a < kDig; ++a) {
+        const float t = rf * 256.0f;
+        const float di = rintf(t);
+        d[a] = di;
+        rf = t - di;
+    }
+}
+
+// BF16 bit pattern of an integer-valued float of magnitude <= 256 (exact)
+__device__ __forceinline__ short bf16_of(float d) { return short(__float_as_uint(d) >> 16); }
+
+// two integer-valued floats (|d| <= 256) as packed BF16: v_cvt_pk_bf16_f32,
+// exact for such values
+__device__ __forceinline__ uint32_t bf16_pair(float a, float b)
+{
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector((fx2){a, b}, bf16x2));
+}
+
+__device__ __forceinline__ bf16x4 bf16_quad(float a, float b, float c, float d)
+{
+    const uint32_t lo = bf16_pair(a, b), hi = bf16_pair(c, d);
+    return __builtin_bit_cast(bf16x4, (uint64_t(hi) << 32) | lo);
+}
+
+__device__ __forceinline__ float wave_max(float x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        x = fmaxf(x, __shfl_xor(x, o));
+    }
+    return x;
+}
+__device__ __forceinline__ double wave_max(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        x = fmax(x, __shfl_xor(x, o));
+    }
+    return x;
+}
+
+// point index (i, j, k) of line index n and cross point x in direction d
+__device__ __forceinline__ int oz_point(int d, int n, int x)
+{
+    return d == 0 ? n * 64 + x : (d == 1 ? (x >> 3) * 64 + n * 8 + (x & 7) : x * 8 + n);
+}
+
+// x[k] for a lane-dependent k by selects (an index into a private array that
+// varies across the lanes would put the array in scratch memory); 0 when k
+// is outside 0 .. K - 1
+template <int K, typename T>
+__device__ __forceinline__ T pick(const T (&x)[K], int k)
+{
+    T r = T(0);
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+        r = k == c ? x[c] : r;
+    }
+    return r;
+}
+
+// The A operands of M (D, or D^T with TR) for this lane, and the scales of
+// its four output rows: lane l holds row (g, i) = l % 32 of A, columns
+// n = 4 h + 0..3 (h = l / 32), and outputs i = 4 h + 0..3.
+struct OzMat {
+    bf16x4 lo[kLoLevels];   // A_a, a = 0..3 (levels 0..3)
+    bf16x4 hi[kDig];        // A'_a, a = 0..5 (levels 4, 5)
+    double scale[4];        // 2^{e_M(i) - 16} of the lane's output rows
+
+    template <bool TR>
+    __device__ void init(const double* __restrict__ D, int lane)
+    {
+        auto M = [&](int i, int n) { return TR ? D[n * kN + i] : D[i * kN + n]; };
+        const int R = lane % 32, g = R / 8, i = R % 8, h = lane / 32;
+        double rowmax[kN];
+#pragma unroll
+        for (int r = 0; r < kN; ++r) {
+            double m = 0.0;
+#pragma unroll
+            for (int n = 0; n < kN; ++n) {
+                m = fmax(m, fabs(M(r, n)));
+            }
+            rowmax[r] = m;
+        }
+        const int ei = exp_of(pick(rowmax, i));
+        double dig[4][kDig];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            digits(ldexp(M(i, 4 * h + t), -ei), dig[t]);
+        }
+#pragma unroll
+        for (int a = 0; a < kLoLevels; ++a) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                lo[a][t] = bf16_of(float(pick(dig[t], g - a)));
+            }
+        }
+#pragma unroll
+        for (int a = 0; a < kDig; ++a) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                hi[a][t] = g < 2 ? bf16_of(float(pick(dig[t], 4 + g - a))) : short(0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            scale[r] = ldexp(1.0, exp_of(h == 0 ? rowmax[r] : rowmax[4 + r]) - 16);
+        }
+    }
+};
+
+// sum_t C_t 2^{-8t} of the lane's output r (C_t in the level accumulators).
+// Levels 5, 4, 3 are combined in FP32 first: in level-3 units their sum is
+// below 2^21, so its FP32 rounding is at most 2^-2 of a level-3 unit, 2^-26
+// of a level-0 unit, below 2^-45 of a result of up to 2^20 level-0 units;
+// levels 2, 1, 0 are added in FP64.
+__device__ __forceinline__ double recombine(const fx16& c_lo, const fx16& c_hi, int r)
+{
+    const float s3 = fmaf(fmaf(c_hi[4 + r], 0x1p-8f, c_hi[r]), 0x1p-8f, c_lo[12 + r]);   // levels 5, 4, 3
+    double s = double(s3);
+#pragma unroll
+    for (int g = 2; g >= 0; --g) {
+        s = fma(s, 0x1p-8, double(c_lo[4 * g + r]));    // levels 2 .. 0
+    }
+    return s;
+}
+
+// out(i, x) for the lane's outputs (i = 4 h + r, x = 32 tile + l % 32) from
+// the digits B[a] of w (packed for this lane: n = 4 h + 0..3)
+__device__ __forceinline__ void oz_contract(const OzMat& M, const bf16x4 (&B)[kDig], double wscale, double (&out)[4])
+{
+    fx16 c_lo = {};
+    fx16 c_hi = {};
+#pragma unroll
+    for (int a = 0; a < kLoLevels; ++a) {
+        c_lo = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(M.lo[a], B[a], c_lo, 0, 0, 0);
+    }
+#pragma unroll
+    for (int a = 0; a < kDig; ++a) {
+        c_hi = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(M.hi[a], B[a], c_hi, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        out[r] = recombine(c_lo, c_hi, r) * (M.scale[r] * wscale);
+    }
+}
+
+// the lane's B operands from its four values w[0..3] (n = 4 h + 0..3),
+// w / 2^{e_w} with |w| / 2^{e_w} < 1
+__device__ __forceinline__ void oz_operands(const float (&w)[4], int ew, bf16x4 (&B)[kDig])
+{
+    float d[4][kDig];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        digits(ldexp(w[t], -ew), d[t]);
+    }
+#pragma unroll
+    for (int a = 0; a < kDig; ++a) {
+        B[a] = bf16_quad(d[0][a], d[1][a], d[2][a], d[3][a]);
+    }
+}
+
+__device__ __forceinline__ void oz_operands(const double (&w)[4], int ew, bf16x4 (&B)[kDig])
+{
+    float d[4][kDig];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        digits_mixed(ldexp(w[t], -ew), d[t]);
+    }
+#pragma unroll
+    for (int a = 0; a < kDig; ++a) {
+        B[a] = bf16_quad(d[0][a], d[1][a], d[2][a], d[3][a]);
+    }
+}
+
+__global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const double* __restrict__ D,
+                                                       const float* __restrict__ G, const float* __restrict__ u,
+                                                       double* __restrict__ v)
+{
+    __shared__ double s_q[3][kN3];   // qr, qs, qt of the element
+    __shared__ double s_v[kN3];      // v of the element
+
+    const int lane = threadIdx.x;
+    const int h = lane / 32;
+    OzMat Mg, Md;
+    Mg.init<false>(D, lane);   // gradient: D(i, n)
+    Md.init<true>(D, lane);    // divergence: D(n, i)
+
+    for (size_t e = blockIdx.x; e < nelmt; e += gridDim.x) {
+        const float* ue = u + e * kN3;
+        const float* ge = G + e * 6 * kN3;
+
+        // one exponent for the element's u
+        float um = 0.0f;
+#pragma unroll
+        for (int t = 0; t < kN; ++t) {
+            um = fmaxf(um, fabsf(ue[lane * kN + t]));
+        }
+        const int eu = exp_of(wave_max(um));
+        const double uscale = ldexp(1.0, eu);
+
+        // gradient: qr, qs, qt (one direction and tile at a time: unrolled,
+        // the loads and digits of all six would be live at once)
+#pragma unroll 1
+        for (int d = 0; d < 3; ++d) {
+#pragma unroll 1
+            for (int tile = 0; tile < 2; ++tile) {
+                const int x = 32 * tile + lane % 32;
+                float w[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    w[t] = ue[oz_point(d, 4 * h + t, x)];
+                }
+                bf16x4 B[kDig];
+                oz_operands(w, eu, B);
+                double q[4];
+                oz_contract(Mg, B, uscale, q);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    s_q[d][oz_point(d, 4 * h + r, x)] = q[r];
+                }
+            }
+        }
+        wave_sync();
+
+        // divergence: per direction, the flux it contracts at its own points
+#pragma unroll 1
+        for (int d = 0; d < 3; ++d) {
+            // factors of the flux of direction d: rr (0, 1, 2), rs (1, 3, 4), rt (2, 4, 5)
+            const int f0 = d == 2 ? 2 : d, f1 = d == 0 ? 1 : (d == 1 ? 3 : 4), f2 = d == 0 ? 2 : (d == 1 ? 4 : 5);
+            double f[2][4];
+            double fm = 0.0;
+#pragma unroll
+            for (int tile = 0; tile < 2; ++tile) {
+                const int x = 32 * tile + lane % 32;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int pt = oz_point(d, 4 * h + t, x);
+                    const int ii = pt >> 6, jj = (pt >> 3) & 7, kk = pt & 7;
+                    const float* gp = ge + (ii * kN + jj) * 6 * kN + kk;
+                    const double val = double(gp[f0 * kN]) * s_q[0][pt] + double(gp[f1 * kN]) * s_q[1][pt] +
+                                       double(gp[f2 * kN]) * s_q[2][pt];
+                    f[tile][t] = val;
+                    fm = fmax(fm, fabs(val));
+                }
+            }
+            const int ef = exp_of(wave_max(fm));
+            const double fscale = ldexp(1.0, ef);
+#pragma unroll 1
+            for (int tile = 0; tile < 2; ++tile) {
+                const int x = 32 * tile + lane % 32;
+                bf16x4 B[kDig];
+                double ft[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    ft[t] = tile == 0 ? f[0][t] : f[1][t];
+                }
+                oz_operands(ft, ef, B);
+                double o[4];
+                oz_contract(Md, B, fscale, o);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int pt = oz_point(d, 4 * h + r, x);
+                    s_v[pt] = d == 0 ? o[r] : s_v[pt] + o[r];
+                }
+            }
+            wave_sync();
+        }
+
+        double* ve = v + e * kN3;
+#pragma unroll
+        for (int m = 0; m < kN; ++m) {
+            st_stream(ve + m * 64 + lane, s_v[m * 64 + lane]);
+        }
+        wave_sync();
+    }
+}
+
+}  // namespace
+
+int bk5_apply_oz(int p, size_t nelmt, const double* D, const float* G, const float* u, double* v, hipStream_t s)
+{
+    if (p != 7) {
+        return BKX_ERROR_ORDER;
+    }
+    const unsigned grid = unsigned(nelmt < size_t(kOzGrid) ? nelmt : size_t(kOzGrid));
+    hipLaunchKernelGGL(bk5_oz_kernel, dim3(grid), dim3(kWave), 0, s, nelmt, D, G, u, v);
+    return launch_status();
+}
+
+}  // namespace bkx

@@ -43,6 +43,7 @@ int geo_bk3_f64(int p, size_t nelmt, const double* B, const double* D, const dou
 int geo_bk5_f64(int p, size_t nelmt, const double* D, const double* wq, const double* coords, double* G,
                 hipStream_t s);
 int bk5_apply_bf16s(int p, size_t nelmt, const double* D, const float* G, const float* u, float* v, hipStream_t s);
+int bk5_apply_oz(int p, size_t nelmt, const double* D, const float* G, const float* u, double* v, hipStream_t s);
 int bk5_variants();
 int bk5_apply_f64(int variant, int p, size_t nelmt, const double* D, const double* G, const double* u, double* v,
                   hipStream_t s);
@@ -474,6 +475,21 @@ int bkx_bk5_apply_bf16s(int p, size_t nelmt, const double* dbasis, const float* 
         return BKX_ERROR_NULL;
     }
     return bkx::bk5_apply_bf16s(p, nelmt, dbasis, G, in, out, as_stream(stream));
+}
+
+int bkx_bk5_apply_oz(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, double* out,
+                     bkx_stream_t stream)
+{
+    if (p != 7) {
+        return BKX_ERROR_ORDER;
+    }
+    if (nelmt == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!dbasis || !G || !in || !out) {
+        return BKX_ERROR_NULL;
+    }
+    return bkx::bk5_apply_oz(p, nelmt, dbasis, G, in, out, as_stream(stream));
 }
 
 int bkx_bk3_apply_kappa_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,

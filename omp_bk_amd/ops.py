@@ -192,6 +192,7 @@ _BK5_FORMS = {
     "f32c": ("bkx_bk5_apply_f32c", torch.float32),
     "f32w": ("bkx_bk5_apply_f32w", torch.float32),
     "bf16s": ("bkx_bk5_apply_bf16s", torch.float32),
+    "oz": ("bkx_bk5_apply_oz", torch.float32),
 }
 
 
@@ -202,7 +203,9 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
          "f32c": u, G in FP32, float-float compensated arithmetic, v FP64;
          "f32w": u, G in FP32, FP64 arithmetic, v FP64;
          "bf16s": u, G in FP32, split-BF16 MFMA contractions accumulated in
-                  FP32 (p = 7 only), v FP32."""
+                  FP32 (p = 7 only), v FP32;
+         "oz": u, G in FP32, contractions from exact 8-bit digit products on
+               BF16 MFMA with exact FP32 level sums (p = 7 only), v FP64."""
     _order_ok(5, p)
     if form not in _BK5_FORMS:
         raise ValueError("unknown BK5 form %r (one of %s)" % (form, ", ".join(_BK5_FORMS)))

@@ -138,3 +138,62 @@ def test_bf16s_full_size_sampled():
     got = out.view(E, -1)[idx].double().cpu().numpy().reshape(-1)
     err = O.rel_maxdiff(got, ref)
     assert 1e-9 < err < 1e-4, err
+
+
+@pytest.mark.parametrize("E", [1, 37, 4097])
+def test_oz_against_quantised_oracle(E):
+    """The MFMA-carried compensated form (six exact 8-bit digits per operand,
+    BF16 MFMA products summed exactly per digit level, FP64 recombination)
+    at p = 7 against the FP64 oracle on the same FP32 inputs, at the
+    FP32-compensated bound of 1e-12; E = 4097 makes some workgroups loop over
+    two elements and one over a partial round."""
+    p = 7
+    D, G, u = _case(p, E, seed=700 + E)
+    ref = O.bk5(p + 1, D, G.astype(np.float32).astype(np.float64), u.astype(np.float32).astype(np.float64))
+    got = _run("oz", p, D, G, u)
+    err = O.rel_maxdiff(got, ref)
+    assert err <= 1e-12, err
+
+
+def test_oz_reference_data_and_zero():
+    """The reference's LCG data on its cos test basis (rows of very different
+    size: every row has its own digit exponent), and zero input (exponent 0),
+    which must give exactly zero."""
+    p, E = 7, 19
+    n = p + 1
+    u, G = O.fill_random(E * n ** 3, E * 6 * n ** 3, seed=77)
+    G = G.astype(np.float32).astype(np.float64)
+    D = O.test_basis(n, n)
+    got = _run("oz", p, D, G, u)
+    assert O.rel_maxdiff(got, O.bk5(n, D, G, u)) <= 1e-12
+    zero = _run("oz", p, D, G, np.zeros_like(u))
+    assert not np.any(zero)
+
+
+def test_oz_full_size_sampled():
+    """p = 7, E = 65536 on the 64x32x32 perturbed mesh: sampled elements
+    against the quantised oracle at 1e-12, and against f32w (the FP64
+    operator on the same FP32 inputs) over the whole output."""
+    p, E = 7, 65536
+    n = p + 1
+    D = torch.as_tensor(geo.bk5_basis(p), device=DEV)
+    G, xyz = geo.geometric_factors(5, p, E, device=DEV, coords=True)
+    G = G.reshape(-1)
+    torch.manual_seed(8)
+    u = (torch.rand(E * n ** 3, dtype=torch.float64, device=DEV) * 2 - 1) + geo.smooth_mode(xyz).reshape(-1)
+    del xyz
+    G32, u32 = G.float(), u.float()
+    del G, u
+    v = ops.bk5(p, D, G32, u32, form="oz")
+    w = ops.bk5(p, D, G32, u32, form="f32w")
+    assert ((v - w).abs().max() / w.abs().max()).item() <= 1e-12
+    idx = torch.as_tensor(np.random.default_rng(9).choice(E, 256, replace=False), device=DEV)
+    ref = O.bk5(n, D.cpu().numpy(), G32.view(E, -1)[idx].double().cpu().numpy(),
+                u32.view(E, -1)[idx].double().cpu().numpy())
+    assert O.rel_maxdiff(v.view(E, -1)[idx].cpu().numpy().reshape(-1), ref) <= 1e-12
+
+
+def test_oz_rejects_other_orders():
+    with pytest.raises(Exception):
+        ops.bk5(6, torch.zeros(49, dtype=torch.float64, device=DEV), torch.zeros(6 * 343, device=DEV),
+                torch.zeros(343, device=DEV), form="oz")
