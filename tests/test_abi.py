@@ -121,3 +121,10 @@ def test_mixed_plan_checks_per_order_arrays():
         mixed.MixedPlan(order, div_by_order=[1] * 8)
     with pytest.raises(ValueError):
         mixed.MixedPlan(order, rounds_by_order=[1] * 10)
+
+
+def test_oz_checks_arguments_without_device():
+    L = _lib.lib()
+    assert L.bkx_bk5_apply_oz(6, 10, None, None, None, None, None) == 1
+    assert L.bkx_bk5_apply_oz(7, 0, None, None, None, None, None) == 0
+    assert L.bkx_bk5_apply_oz(7, 10, None, None, None, None, None) == 2
