@@ -375,7 +375,9 @@ def adjoint_config(steps, warmup, p=6, E=16384, forms=("f64", "f32w"), ms64=None
 def mixed_batch(steps, warmup, nelmt=200000):
     """BK3 over 200,000 elements of mixed order (Zipf(1.2) over p = 1..8):
     the single launch with divisors tuned per order (one workgroup per
-    tile), the per-order launches, and the single launch as a persistent
+    tile), the per-order launches with each order's launch shape (divisor
+    1, 2 or 4, rows of G in flight 2, 3 or 4) tuned, and the single launch
+    as a persistent
     loop over the tiles (one workgroup per CU slot), each replayed from a
     hipGraph; the first two also with the bases in even-odd form (_eo); and
     the same elements in the grouped layout (_grouped: each order's elements
@@ -392,7 +394,8 @@ def mixed_batch(steps, warmup, nelmt=200000):
     u = torch.rand(plan0.ndof, dtype=torch.float64, device=dev) * 2 - 1
     v = torch.empty_like(u)
     div1, _ = mixed.tune_single(order, bases, G, u, v, steps=5)
-    divp, _ = mixed.tune(order, bases, G, u, v, steps=5)
+    shapes, _ = mixed.tune(order, bases, G, u, v, steps=5, plan=plan0)
+    divp = [1] * 9
     nbytes = 8 * (2 * plan0.ndof + plan0.ngeo)
     out = {"nelmt": nelmt, "counts_by_p": np.bincount(order, minlength=9)[1:].tolist(), "bytes": nbytes}
     P = mixed.prepared_bases(bases)
@@ -402,6 +405,7 @@ def mixed_batch(steps, warmup, nelmt=200000):
     Gg = mixed.affine_factors(order, gplan0.goff, gplan0.ngeo, device=dev)
     ug = torch.rand(gplan0.ndof, dtype=torch.float64, device=dev) * 2 - 1
     vg = torch.empty_like(ug)
+    shapes_g, _ = mixed.tune(order, bases, Gg, ug, vg, steps=5, plan=gplan0)
     for name, by_order, div, grid, prep, grouped in (
             ("single_launch_graph", False, div1, 0, False, False),
             ("per_order_graph", True, divp, 0, False, False),
@@ -414,12 +418,19 @@ def mixed_batch(steps, warmup, nelmt=200000):
             ("single_launch_graph_grouped_eo", False, div1, 0, True, True),
             ("per_order_graph_grouped_eo", True, divp, 0, True, True)):
         plan = mixed.MixedPlan(order, div_by_order=div, grid=grid, grouped=grouped)
+        if by_order:
+            for p, shape in (shapes_g if grouped else shapes).items():
+                plan.set_order_shape(p, *shape)
         arrays = (Gg, ug, vg) if grouped else (G, u, v)
         g = plan.graph(P if prep else bases, *arrays, by_order=by_order, prepared=prep)
         ms, _ = time_steps(g.launch, steps, warmup)
         g.close()
         out[name] = {"ms": round(ms, 4), "GDoF/s": round(plan.dofs / (ms * 1e-3) * 1e-9, 3),
-                     "GB/s": round(nbytes / (ms * 1e-3) * 1e-9, 1), "div_by_p": div[1:]}
+                     "GB/s": round(nbytes / (ms * 1e-3) * 1e-9, 1)}
+        if by_order:
+            out[name]["shape_by_p"] = {p: list(plan.order_shape(p)) for p in range(1, 9)}
+        else:
+            out[name]["div_by_p"] = div[1:]
         if not by_order:
             out[name]["ntiles"] = plan.ntiles
         plan.close()

@@ -371,6 +371,15 @@ int bkx_mixed_plan_destroy(bkx_mixed_plan* plan);
 int bkx_mixed_plan_info(const bkx_mixed_plan* plan, size_t* ndof, size_t* ngeo, int* ntiles, int* grid);
 /* The element offsets uoff, goff (host arrays of nelmt). */
 int bkx_mixed_plan_offsets(const bkx_mixed_plan* plan, int64_t* uoff, int64_t* goff);
+/* The launch shape of order p's kernel in the per-order launches
+ * (bkx_bk3_mixed_apply_by_order*, _apply_order, by-order graphs made after
+ * the call): div, the elements-per-workgroup divisor of floor(256 / q^2)
+ * (1, 2 or 4), and gpf, the rows of G kept in flight (2, 3 or 4); the
+ * defaults are div_by_order[p] and 3.  The tuner (mixed.tune) searches them
+ * per order.  The results are the same bits for every shape.  Host-side
+ * only; BKX_ERROR_ARG for another div or gpf. */
+int bkx_mixed_plan_set_order_shape(bkx_mixed_plan* plan, int p, int div, int gpf);
+int bkx_mixed_plan_get_order_shape(const bkx_mixed_plan* plan, int p, int* div, int* gpf);
 
 /* One launch over every tile: one workgroup per tile (default), or
  * persistent when the plan was made so (see bkx_mixed_plan_create).  The

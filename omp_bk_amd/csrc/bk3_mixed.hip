@@ -176,10 +176,13 @@ __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
 // variants (with their register budget: at least four waves per SIMD).  Workgroup k runs tile k of the order's range, looping when the
 // tile holds more than one workgroup's elements.  The arithmetic is the
 // all-order kernel's: the two give the same bits.
-constexpr int kOrderGpf = 3;
+constexpr int kOrderGpf = 3;   // the default rows of G in flight
 constexpr int kOrderNt = 1;
 
-template <int P, int DIV, bool EO>
+// GPF: rows of G in flight (2, 3 or 4), DIV: elements-per-workgroup divisor
+// (1, 2 or 4); the launch shape is chosen per order by the tuner
+// (bkx_mixed_plan_set_order_shape).
+template <int P, int DIV, bool EO, int GPF = kOrderGpf>
 __global__ __launch_bounds__((bk13t::Bk13tShape<P + 1, P + 2, DIV>::threads))
 __attribute__((amdgpu_waves_per_eu(4))) void bk3_order_kernel(
     int first, int count, const int64_t* __restrict__ uoff, const int64_t* __restrict__ goff,
@@ -195,7 +198,8 @@ __attribute__((amdgpu_waves_per_eu(4))) void bk3_order_kernel(
     const int idx = int(blockIdx.x) * S::epb + slot;
     const bool active = slot < S::epb && idx < count;
     const int e = first + (active ? idx : 0);
-    bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kOrderGpf, BKX_FORM_F64, double, kOrderNt, EO>(
+    constexpr int gpf = GPF < NQ ? GPF : NQ;   // at most a whole element's rows
+    bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, gpf, BKX_FORM_F64, double, kOrderNt, EO>(
         B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l, smem);
 }
 
@@ -214,6 +218,8 @@ struct bkx_mixed_plan {
     bool persistent = false;    // loop over tiles (grid resident workgroups) or one workgroup per tile
     int elem_begin[10] = {};            // per order: range of the order-sorted element list
     int div_p[10] = {};                 // per order: elements-per-workgroup divisor
+    int order_div[10] = {};             // per order: the per-order launch's divisor (1, 2, 4)
+    int order_gpf[10] = {};             // per order: the per-order launch's rows of G in flight (2, 3, 4)
     int64_t boff_h[10] = {};            // per order: offset of B_p in the bases array
     int64_t eoff_h[10] = {};            // per order: offset of B_p's even-odd form in the prepared array
     std::vector<int64_t> uoff_h, goff_h;
@@ -262,7 +268,7 @@ int launch_tiles(const bkx::MixedTile* tiles, int ntiles, bool persistent, const
     return bkx::launch_status();
 }
 
-template <int P, int DIV, bool EO>
+template <int P, int DIV, bool EO, int GPF>
 void launch_order_div(int first, int count, const bkx_mixed_plan* pl, const double* bases, const double* G,
                       const double* in, double* out, hipStream_t s)
 {
@@ -270,8 +276,21 @@ void launch_order_div(int first, int count, const bkx_mixed_plan* pl, const doub
     const double* B = bases + (EO ? pl->eoff_h[P] : pl->boff_h[P]);
     const double* D = B + (EO ? bkx::EoMat<P + 1, P + 2, 1>::size : (P + 1) * (P + 2));
     const unsigned grid = unsigned((count + S::epb - 1) / S::epb);
-    hipLaunchKernelGGL((bkx::bk3_order_kernel<P, DIV, EO>), dim3(grid), dim3(S::threads), 0, s, first, count,
+    hipLaunchKernelGGL((bkx::bk3_order_kernel<P, DIV, EO, GPF>), dim3(grid), dim3(S::threads), 0, s, first, count,
                        pl->uoff, pl->goff, B, D, G, in, out);
+}
+
+template <int P, int DIV, bool EO>
+void launch_order_gpf(int gpf, int first, int count, const bkx_mixed_plan* pl, const double* bases,
+                      const double* G, const double* in, double* out, hipStream_t s)
+{
+    if (gpf == 2) {
+        launch_order_div<P, DIV, EO, 2>(first, count, pl, bases, G, in, out, s);
+    } else if (gpf == 4) {
+        launch_order_div<P, DIV, EO, 4>(first, count, pl, bases, G, in, out, s);
+    } else {
+        launch_order_div<P, DIV, EO, 3>(first, count, pl, bases, G, in, out, s);
+    }
 }
 
 // One order's elements, by that order's own kernel: workgroup k takes
@@ -285,13 +304,16 @@ int launch_order(int p, const bkx_mixed_plan* pl, const double* bases, const dou
     if (count == 0) {
         return BKX_SUCCESS;
     }
-    const bool half = pl->div_p[p] == 2;
+    const int div = pl->order_div[p];
+    const int gpf = pl->order_gpf[p];
     return bkx::with_order<1, 8>(p, [&](auto P) {
         constexpr int PP = decltype(P)::value;
-        if (half) {
-            launch_order_div<PP, 2, EO>(first, count, pl, bases, G, in, out, s);
+        if (div == 4) {
+            launch_order_gpf<PP, 4, EO>(gpf, first, count, pl, bases, G, in, out, s);
+        } else if (div == 2) {
+            launch_order_gpf<PP, 2, EO>(gpf, first, count, pl, bases, G, in, out, s);
         } else {
-            launch_order_div<PP, 1, EO>(first, count, pl, bases, G, in, out, s);
+            launch_order_gpf<PP, 1, EO>(gpf, first, count, pl, bases, G, in, out, s);
         }
         return bkx::launch_status();
     });
@@ -474,6 +496,8 @@ int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
         }
         const int per_tile = int(std::min<int64_t>(int64_t(epb) * rounds, INT32_MAX));
         pl->div_p[p] = div;
+        pl->order_div[p] = div;
+        pl->order_gpf[p] = bkx::kOrderGpf;
         const int first = int(elems.size());
         pl->elem_begin[p] = first;
         elems.insert(elems.end(), by_p[p].begin(), by_p[p].end());
@@ -594,6 +618,35 @@ int bkx_mixed_plan_info(const bkx_mixed_plan* plan, size_t* ndof, size_t* ngeo, 
     if (grid) {
         *grid = plan->grid;
     }
+    return BKX_SUCCESS;
+}
+
+int bkx_mixed_plan_set_order_shape(bkx_mixed_plan* plan, int p, int div, int gpf)
+{
+    if (!plan) {
+        return BKX_ERROR_NULL;
+    }
+    if (p < 1 || p > 8) {
+        return BKX_ERROR_ORDER;
+    }
+    if ((div != 1 && div != 2 && div != 4) || gpf < 2 || gpf > 4) {
+        return BKX_ERROR_ARG;
+    }
+    plan->order_div[p] = div;
+    plan->order_gpf[p] = gpf;
+    return BKX_SUCCESS;
+}
+
+int bkx_mixed_plan_get_order_shape(const bkx_mixed_plan* plan, int p, int* div, int* gpf)
+{
+    if (!plan || !div || !gpf) {
+        return BKX_ERROR_NULL;
+    }
+    if (p < 1 || p > 8) {
+        return BKX_ERROR_ORDER;
+    }
+    *div = plan->order_div[p];
+    *gpf = plan->order_gpf[p];
     return BKX_SUCCESS;
 }
 

@@ -38,6 +38,9 @@ def _bind():
     L.bkx_mixed_plan_info.argtypes = [_vp, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t),
                                       ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.bkx_mixed_plan_offsets.argtypes = [_vp, _i64p, _i64p]
+    L.bkx_mixed_plan_set_order_shape.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.bkx_mixed_plan_get_order_shape.argtypes = [_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                                 ctypes.POINTER(ctypes.c_int)]
     L.bkx_bk3_mixed_apply_f64.argtypes = [_vp] + [_vp] * 5
     L.bkx_bk3_mixed_apply_by_order_f64.argtypes = [_vp] + [_vp] * 5
     L.bkx_bk3_mixed_apply_order_f64.argtypes = [_vp, ctypes.c_int] + [_vp] * 5
@@ -51,6 +54,7 @@ def _bind():
     L.bkx_graph_launch.argtypes = [_vp, _vp]
     L.bkx_graph_destroy.argtypes = [_vp]
     for name in ("bkx_mixed_plan_create", "bkx_mixed_plan_create_grouped", "bkx_mixed_plan_destroy", "bkx_mixed_plan_info", "bkx_mixed_plan_offsets",
+                 "bkx_mixed_plan_set_order_shape", "bkx_mixed_plan_get_order_shape",
                  "bkx_bk3_mixed_apply_f64", "bkx_bk3_mixed_apply_by_order_f64", "bkx_bk3_mixed_apply_order_f64",
                  "bkx_bk3_mixed_graph_create", "bkx_graph_launch", "bkx_graph_destroy", "bkx_bk3_mixed_prepare",
                  "bkx_bk3_mixed_apply_prepared_f64", "bkx_bk3_mixed_apply_by_order_prepared_f64",
@@ -194,6 +198,18 @@ class MixedPlan:
         _check("bkx_mixed_plan_offsets", L.bkx_mixed_plan_offsets(h, self.uoff.ctypes.data_as(_i64p),
                                                                   self.goff.ctypes.data_as(_i64p)))
 
+    def set_order_shape(self, p, div, gpf):
+        """Launch shape of order p's kernel in the per-order launches:
+        elements-per-workgroup divisor div (1, 2, 4) and rows of G in flight
+        gpf (2, 3, 4)."""
+        _check("bkx_mixed_plan_set_order_shape", _bind().bkx_mixed_plan_set_order_shape(self.handle, p, div, gpf))
+
+    def order_shape(self, p):
+        d, g = ctypes.c_int(), ctypes.c_int()
+        _check("bkx_mixed_plan_get_order_shape",
+               _bind().bkx_mixed_plan_get_order_shape(self.handle, p, ctypes.byref(d), ctypes.byref(g)))
+        return d.value, g.value
+
     def close(self):
         if self.handle:
             _bind().bkx_mixed_plan_destroy(self.handle)
@@ -257,6 +273,18 @@ class MixedGraph:
     def launch(self):
         _check("bkx_graph_launch", _bind().bkx_graph_launch(self.handle, torch.cuda.current_stream().cuda_stream))
 
+    def set_order_shape(self, p, div, gpf):
+        """Launch shape of order p's kernel in the per-order launches:
+        elements-per-workgroup divisor div (1, 2, 4) and rows of G in flight
+        gpf (2, 3, 4)."""
+        _check("bkx_mixed_plan_set_order_shape", _bind().bkx_mixed_plan_set_order_shape(self.handle, p, div, gpf))
+
+    def order_shape(self, p):
+        d, g = ctypes.c_int(), ctypes.c_int()
+        _check("bkx_mixed_plan_get_order_shape",
+               _bind().bkx_mixed_plan_get_order_shape(self.handle, p, ctypes.byref(d), ctypes.byref(g)))
+        return d.value, g.value
+
     def close(self):
         if self.handle:
             _bind().bkx_graph_destroy(self.handle)
@@ -305,20 +333,30 @@ def _batch_ms(fn, steps=10, reps=10, warmup=2):
     return statistics.median(out)
 
 
-def tune(order, bases, G, u, v, steps=10):
-    """For the per-order launches: per order, the elements-per-workgroup
-    divisor (1 or 2) whose launch of that order's elements (by the kernel
-    compiled for that order) is fastest, timed back to back as in the
-    per-order sequence; returns (div_by_order, {p: {div: ms}})."""
-    plans = {d: MixedPlan(order, div_by_order=[d] * 9) for d in (1, 2)}
-    best = [1] * 9
-    times = {}
+ORDER_SHAPES = [(d, g) for d in (1, 2, 4) for g in (2, 3, 4)]
+
+
+def tune(order, bases, G, u, v, steps=10, plan=None):
+    """For the per-order launches: per order, the launch shape (elements-
+    per-workgroup divisor 1, 2 or 4, and rows of G in flight 2, 3 or 4)
+    whose launch of that order's elements (by the kernel compiled for that
+    order) is fastest, timed back to back as in the per-order sequence.
+    With plan (the plan whose layout u, v and G follow), the shapes are
+    timed on it and set on it; else on a plan of the caller's layout.
+    Returns ({p: (div, gpf)}, {p: {(div, gpf): ms}})."""
+    own = plan is None
+    pl = MixedPlan(order) if own else plan
+    best, times = {}, {}
     for p in range(1, 9):
         if not np.any(order == p):
             continue
-        times[p] = {d: _batch_ms(lambda: pl.apply_order(p, bases, G, u, v), steps) for d, pl in plans.items()}
+        times[p] = {}
+        for div, gpf in ORDER_SHAPES:
+            pl.set_order_shape(p, div, gpf)
+            times[p][(div, gpf)] = _batch_ms(lambda: pl.apply_order(p, bases, G, u, v), steps)
         best[p] = min(times[p], key=times[p].get)
-    for pl in plans.values():
+        pl.set_order_shape(p, *best[p])
+    if own:
         pl.close()
     return best, times
 

@@ -128,3 +128,11 @@ def test_oz_checks_arguments_without_device():
     assert L.bkx_bk5_apply_oz(6, 10, None, None, None, None, None) == 1
     assert L.bkx_bk5_apply_oz(7, 0, None, None, None, None, None) == 0
     assert L.bkx_bk5_apply_oz(7, 10, None, None, None, None, None) == 2
+
+
+def test_order_shape_checks_without_device():
+    from omp_bk_amd import mixed
+    M = mixed._bind()
+    d, g = ctypes.c_int(), ctypes.c_int()
+    assert M.bkx_mixed_plan_set_order_shape(None, 3, 1, 3) == 2
+    assert M.bkx_mixed_plan_get_order_shape(None, 3, ctypes.byref(d), ctypes.byref(g)) == 2

@@ -69,16 +69,45 @@ def test_small_batches(nelmt):
     assert torch.equal(v1, v3)
 
 
-def test_tuned_divisors():
+def test_tuned_order_shapes():
+    """The per-order tuner sets a launch shape (divisor, rows of G in
+    flight) per order on the plan; the tuned launches match the oracle."""
     order, plan, bases, G, u = _batch(3000, seed=5)
     b, g, uu = _t(bases), _t(G), _t(u)
     v = torch.empty_like(uu)
-    div, times = mixed.tune(order, b, g, uu, v, steps=3)
-    tuned = mixed.MixedPlan(order, div_by_order=div)
+    shapes, times = mixed.tune(order, b, g, uu, v, steps=3, plan=plan)
+    assert set(shapes) == set(int(p) for p in np.unique(order))
+    for p, shape in shapes.items():
+        assert shape in mixed.ORDER_SHAPES and plan.order_shape(p) == shape
+        assert len(times[p]) == len(mixed.ORDER_SHAPES)
     vt = torch.full_like(uu, float("nan"))
-    tuned.apply(b, g, uu, vt)
+    plan.apply_by_order(b, g, uu, vt)
     torch.cuda.synchronize()
-    _oracle_check(order, tuned, bases, G, u, vt.cpu().numpy())
+    _oracle_check(order, plan, bases, G, u, vt.cpu().numpy())
+
+
+def test_every_order_shape_same_bits():
+    """Every per-order launch shape gives the bits of the default shape,
+    also with the bases in even-odd form."""
+    order, plan, bases, G, u = _batch(2000, seed=15)
+    b, g, uu = _t(bases), _t(G), _t(u)
+    P = mixed.prepared_bases(b)
+    ref = torch.full_like(uu, float("nan"))
+    plan.apply_by_order(b, g, uu, ref)
+    ref_eo = torch.full_like(uu, float("nan"))
+    plan.apply_by_order(P, g, uu, ref_eo, prepared=True)
+    for div, gpf in mixed.ORDER_SHAPES:
+        for p in range(1, 9):
+            plan.set_order_shape(p, div, gpf)
+        v = torch.full_like(uu, float("nan"))
+        plan.apply_by_order(b, g, uu, v)
+        v_eo = torch.full_like(uu, float("nan"))
+        plan.apply_by_order(P, g, uu, v_eo, prepared=True)
+        torch.cuda.synchronize()
+        assert torch.equal(v, ref), (div, gpf)
+        assert torch.equal(v_eo, ref_eo), (div, gpf)
+    with pytest.raises(RuntimeError):
+        plan.set_order_shape(3, 3, 3)
 
 
 def test_tuned_single_launch_divisors():
