@@ -125,7 +125,9 @@ int bkx_bk5_apply_f32w(int p, size_t nelmt, const double* dbasis, const float* G
 int bkx_bk5_apply_bf16s(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, float* out,
                         bkx_stream_t stream);
 
-/* FP32-compensated on the matrix cores, p = 7 only: every contraction is
+/* FP32-compensated on the matrix cores, p = 5, 6, 7 (n = 6, 7, 8 points per
+ * direction, where the n x n contractions fill the MFMA tiles; the tiles are
+ * 8 deep, so n < 8 is zero-padded; BKX_ERROR_ORDER otherwise): every contraction is
  * evaluated from six 8-bit integer digits of each operand (D per row, u and
  * each flux field per element, each digit exact in BF16) whose products are
  * summed per digit level by v_mfma_f32_32x32x8bf16_1k, exactly (every level

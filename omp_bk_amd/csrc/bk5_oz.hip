@@ -1,4 +1,4 @@
-// bk5_oz.hip -- BK5 collocated Laplacian at p = 7 with FP32 storage and
+// bk5_oz.hip -- BK5 collocated Laplacian at p = 5, 6, 7 with FP32 storage and
 // its contractions carried by BF16 MFMA products of exact integer digits
 // (an Ozaki-style error-free splitting; no reference counterpart: the
 // reference runs BK5 in float or double, openmp/BK5.cpp:18-103).
@@ -24,7 +24,9 @@
 //
 // MFMA mapping (v_mfma_f32_32x32x8bf16_1k, one wavefront per element): rows
 // of A and C are (level group g, output i), 4 x 8; columns of B and C are 32
-// cross points x; K = 8 is the contraction index n.  Lane l holds
+// cross points x (two tiles cover the n^2 <= 64 cross points); K = 8 is the
+// contraction index n.  For n = 6, 7 (p = 5, 6) the rows, columns and K
+// beyond n carry zero digits and their outputs are not stored.  Lane l holds
 // A[l % 32][4 (l / 32) + 0..3], B[4 (l / 32) + 0..3][l % 32] and
 // C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32], r = 0..15, so that every lane
 // holds all level groups of its four outputs: the recombination is local to
@@ -59,8 +61,7 @@ using fx16 = float __attribute__((ext_vector_type(16)));
 using bf16x2 = __bf16 __attribute__((ext_vector_type(2)));
 using fx2 = float __attribute__((ext_vector_type(2)));
 
-constexpr int kN = 8;          // points per direction (p = 7)
-constexpr int kN3 = 512;
+constexpr int kK = 8;          // MFMA K and output rows: points per direction, padded
 constexpr int kDig = 6;        // digits per operand
 constexpr int kLoLevels = 4;   // levels 0..3 in the first accumulator
 constexpr int kOzGrid = 4096;  // workgroups (one wavefront each), looping over elements
@@ -154,10 +155,12 @@ __device__ __forceinline__ double wave_max(double x)
     return x;
 }
 
-// point index (i, j, k) of line index n and cross point x in direction d
+// point index (i, j, k) of line index n and cross point x = (a, b) in
+// direction d, N points per direction
+template <int N>
 __device__ __forceinline__ int oz_point(int d, int n, int x)
 {
-    return d == 0 ? n * 64 + x : (d == 1 ? (x >> 3) * 64 + n * 8 + (x & 7) : x * 8 + n);
+    return d == 0 ? n * N * N + x : (d == 1 ? (x / N) * N * N + n * N + x % N : x * N + n);
 }
 
 // x[k] for a lane-dependent k by selects (an index into a private array that
@@ -182,17 +185,18 @@ struct OzMat {
     bf16x4 hi[kDig];        // A'_a, a = 0..5 (levels 4, 5)
     double scale[4];        // 2^{e_M(i) - 16} of the lane's output rows
 
-    template <bool TR>
+    template <bool TR, int N>
     __device__ void init(const double* __restrict__ D, int lane)
     {
-        auto M = [&](int i, int n) { return TR ? D[n * kN + i] : D[i * kN + n]; };
+        // M padded with zeros to 8 x 8
+        auto M = [&](int i, int n) { return (i < N && n < N) ? (TR ? D[n * N + i] : D[i * N + n]) : 0.0; };
         const int R = lane % 32, g = R / 8, i = R % 8, h = lane / 32;
-        double rowmax[kN];
+        double rowmax[kK];
 #pragma unroll
-        for (int r = 0; r < kN; ++r) {
+        for (int r = 0; r < kK; ++r) {
             double m = 0.0;
 #pragma unroll
-            for (int n = 0; n < kN; ++n) {
+            for (int n = 0; n < kK; ++n) {
                 m = fmax(m, fabs(M(r, n)));
             }
             rowmax[r] = m;
@@ -288,34 +292,42 @@ __device__ __forceinline__ void oz_operands(const double (&w)[4], int ew, bf16x4
     }
 }
 
+template <int N>
 __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const double* __restrict__ D,
                                                        const float* __restrict__ G, const float* __restrict__ u,
                                                        double* __restrict__ v)
 {
-    __shared__ double s_q[3][kN3];   // qr, qs, qt of the element
-    __shared__ double s_v[kN3];      // v of the element
+    constexpr int N2 = N * N;
+    constexpr int N3 = N2 * N;
+    constexpr bool kFull = N == kK;   // no padding: every mask below is true
+    __shared__ double s_q[3][N3];   // qr, qs, qt of the element
+    __shared__ double s_v[N3];      // v of the element
 
     const int lane = threadIdx.x;
     const int h = lane / 32;
     OzMat Mg, Md;
-    Mg.init<false>(D, lane);   // gradient: D(i, n)
-    Md.init<true>(D, lane);    // divergence: D(n, i)
+    Mg.init<false, N>(D, lane);   // gradient: D(i, n)
+    Md.init<true, N>(D, lane);    // divergence: D(n, i)
 
     for (size_t e = blockIdx.x; e < nelmt; e += gridDim.x) {
-        const float* ue = u + e * kN3;
-        const float* ge = G + e * 6 * kN3;
+        const float* ue = u + e * N3;
+        const float* ge = G + e * 6 * N3;
 
         // one exponent for the element's u
         float um = 0.0f;
 #pragma unroll
-        for (int t = 0; t < kN; ++t) {
-            um = fmaxf(um, fabsf(ue[lane * kN + t]));
+        for (int m = 0; m < (N3 + kWave - 1) / kWave; ++m) {
+            const int x = m * kWave + lane;
+            um = fmaxf(um, (kFull || x < N3) ? fabsf(ue[(kFull || x < N3) ? x : 0]) : 0.0f);
         }
         const int eu = exp_of(wave_max(um));
         const double uscale = ldexp(1.0, eu);
 
         // gradient: qr, qs, qt (one direction and tile at a time: unrolled,
-        // the loads and digits of all six would be live at once)
+        // the loads and digits of all six would be live at once).  Line
+        // indices n >= N and cross points x >= N^2 are padding: their values
+        // are zero (loaded from a valid address and masked) and their
+        // outputs are not stored.
 #pragma unroll 1
         for (int d = 0; d < 3; ++d) {
 #pragma unroll 1
@@ -324,7 +336,9 @@ __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const doubl
                 float w[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    w[t] = ue[oz_point(d, 4 * h + t, x)];
+                    const bool ok = kFull || (4 * h + t < N && x < N2);
+                    const float val = ue[ok ? oz_point<N>(d, 4 * h + t, x) : 0];
+                    w[t] = ok ? val : 0.0f;
                 }
                 bf16x4 B[kDig];
                 oz_operands(w, eu, B);
@@ -332,7 +346,9 @@ __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const doubl
                 oz_contract(Mg, B, uscale, q);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    s_q[d][oz_point(d, 4 * h + r, x)] = q[r];
+                    if (kFull || (4 * h + r < N && x < N2)) {
+                        s_q[d][oz_point<N>(d, 4 * h + r, x)] = q[r];
+                    }
                 }
             }
         }
@@ -350,13 +366,14 @@ __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const doubl
                 const int x = 32 * tile + lane % 32;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    const int pt = oz_point(d, 4 * h + t, x);
-                    const int ii = pt >> 6, jj = (pt >> 3) & 7, kk = pt & 7;
-                    const float* gp = ge + (ii * kN + jj) * 6 * kN + kk;
-                    const double val = double(gp[f0 * kN]) * s_q[0][pt] + double(gp[f1 * kN]) * s_q[1][pt] +
-                                       double(gp[f2 * kN]) * s_q[2][pt];
-                    f[tile][t] = val;
-                    fm = fmax(fm, fabs(val));
+                    const bool ok = kFull || (4 * h + t < N && x < N2);
+                    const int pt = ok ? oz_point<N>(d, 4 * h + t, x) : 0;
+                    const int ii = pt / N2, jj = (pt / N) % N, kk = pt % N;
+                    const float* gp = ge + (ii * N + jj) * 6 * N + kk;
+                    const double val = double(gp[f0 * N]) * s_q[0][pt] + double(gp[f1 * N]) * s_q[1][pt] +
+                                       double(gp[f2 * N]) * s_q[2][pt];
+                    f[tile][t] = ok ? val : 0.0;
+                    fm = fmax(fm, fabs(f[tile][t]));
                 }
             }
             const int ef = exp_of(wave_max(fm));
@@ -375,17 +392,22 @@ __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const doubl
                 oz_contract(Md, B, fscale, o);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int pt = oz_point(d, 4 * h + r, x);
-                    s_v[pt] = d == 0 ? o[r] : s_v[pt] + o[r];
+                    if (kFull || (4 * h + r < N && x < N2)) {
+                        const int pt = oz_point<N>(d, 4 * h + r, x);
+                        s_v[pt] = d == 0 ? o[r] : s_v[pt] + o[r];
+                    }
                 }
             }
             wave_sync();
         }
 
-        double* ve = v + e * kN3;
+        double* ve = v + e * N3;
 #pragma unroll
-        for (int m = 0; m < kN; ++m) {
-            st_stream(ve + m * 64 + lane, s_v[m * 64 + lane]);
+        for (int m = 0; m < (N3 + kWave - 1) / kWave; ++m) {
+            const int x = m * kWave + lane;
+            if (kFull || x < N3) {
+                st_stream(ve + x, s_v[x]);
+            }
         }
         wave_sync();
     }
@@ -395,12 +417,21 @@ __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const doubl
 
 int bk5_apply_oz(int p, size_t nelmt, const double* D, const float* G, const float* u, double* v, hipStream_t s)
 {
-    if (p != 7) {
+    const unsigned grid = unsigned(nelmt < size_t(kOzGrid) ? nelmt : size_t(kOzGrid));
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kWave), 0, s, nelmt, D, G, u, v);
+        return launch_status();
+    };
+    switch (p) {
+    case 5:
+        return go(bk5_oz_kernel<6>);
+    case 6:
+        return go(bk5_oz_kernel<7>);
+    case 7:
+        return go(bk5_oz_kernel<8>);
+    default:
         return BKX_ERROR_ORDER;
     }
-    const unsigned grid = unsigned(nelmt < size_t(kOzGrid) ? nelmt : size_t(kOzGrid));
-    hipLaunchKernelGGL(bk5_oz_kernel, dim3(grid), dim3(kWave), 0, s, nelmt, D, G, u, v);
-    return launch_status();
 }
 
 }  // namespace bkx

@@ -480,7 +480,7 @@ int bkx_bk5_apply_bf16s(int p, size_t nelmt, const double* dbasis, const float* 
 int bkx_bk5_apply_oz(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, double* out,
                      bkx_stream_t stream)
 {
-    if (p != 7) {
+    if (p < 5 || p > 7) {
         return BKX_ERROR_ORDER;
     }
     if (nelmt == 0) {

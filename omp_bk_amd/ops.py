@@ -205,7 +205,7 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
          "bf16s": u, G in FP32, split-BF16 MFMA contractions accumulated in
                   FP32 (p = 7 only), v FP32;
          "oz": u, G in FP32, contractions from exact 8-bit digit products on
-               BF16 MFMA with exact FP32 level sums (p = 7 only), v FP64."""
+               BF16 MFMA with exact FP32 level sums (p = 5, 6, 7), v FP64."""
     _order_ok(5, p)
     if form not in _BK5_FORMS:
         raise ValueError("unknown BK5 form %r (one of %s)" % (form, ", ".join(_BK5_FORMS)))

@@ -125,7 +125,8 @@ def test_mixed_plan_checks_per_order_arrays():
 
 def test_oz_checks_arguments_without_device():
     L = _lib.lib()
-    assert L.bkx_bk5_apply_oz(6, 10, None, None, None, None, None) == 1
+    assert L.bkx_bk5_apply_oz(4, 10, None, None, None, None, None) == 1
+    assert L.bkx_bk5_apply_oz(6, 10, None, None, None, None, None) == 2
     assert L.bkx_bk5_apply_oz(7, 0, None, None, None, None, None) == 0
     assert L.bkx_bk5_apply_oz(7, 10, None, None, None, None, None) == 2
 

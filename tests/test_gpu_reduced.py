@@ -140,26 +140,27 @@ def test_bf16s_full_size_sampled():
     assert 1e-9 < err < 1e-4, err
 
 
+@pytest.mark.parametrize("p", [5, 6, 7])
 @pytest.mark.parametrize("E", [1, 37, 4097])
-def test_oz_against_quantised_oracle(E):
+def test_oz_against_quantised_oracle(p, E):
     """The MFMA-carried compensated form (six exact 8-bit digits per operand,
     BF16 MFMA products summed exactly per digit level, FP64 recombination)
-    at p = 7 against the FP64 oracle on the same FP32 inputs, at the
-    FP32-compensated bound of 1e-12; E = 4097 makes some workgroups loop over
-    two elements and one over a partial round."""
-    p = 7
-    D, G, u = _case(p, E, seed=700 + E)
+    at p = 5, 6, 7 (n = 6, 7 zero-padded to the 8-deep tiles) against the
+    FP64 oracle on the same FP32 inputs, at the FP32-compensated bound of
+    1e-12; E = 4097 makes some workgroups loop over two elements."""
+    D, G, u = _case(p, E, seed=700 + 10 * p + E)
     ref = O.bk5(p + 1, D, G.astype(np.float32).astype(np.float64), u.astype(np.float32).astype(np.float64))
     got = _run("oz", p, D, G, u)
     err = O.rel_maxdiff(got, ref)
     assert err <= 1e-12, err
 
 
-def test_oz_reference_data_and_zero():
+@pytest.mark.parametrize("p", [5, 7])
+def test_oz_reference_data_and_zero(p):
     """The reference's LCG data on its cos test basis (rows of very different
     size: every row has its own digit exponent), and zero input (exponent 0),
     which must give exactly zero."""
-    p, E = 7, 19
+    E = 19
     n = p + 1
     u, G = O.fill_random(E * n ** 3, E * 6 * n ** 3, seed=77)
     G = G.astype(np.float32).astype(np.float64)
@@ -195,5 +196,5 @@ def test_oz_full_size_sampled():
 
 def test_oz_rejects_other_orders():
     with pytest.raises(Exception):
-        ops.bk5(6, torch.zeros(49, dtype=torch.float64, device=DEV), torch.zeros(6 * 343, device=DEV),
-                torch.zeros(343, device=DEV), form="oz")
+        ops.bk5(4, torch.zeros(25, dtype=torch.float64, device=DEV), torch.zeros(6 * 125, device=DEV),
+                torch.zeros(125, device=DEV), form="oz")
