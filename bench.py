@@ -41,7 +41,7 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk5_bf16s", "bk5_oz", "bk3", "copy", "mixed",
                                        "adjoint_f64", "adjoint_f32w", "adjoint_f32c", "adjoint_f64_eo",
@@ -51,13 +51,27 @@ def parse():
     return ap.parse_args()
 
 
-def time_steps(fn, steps, warmup):
+# Untimed work before each line's timed steps, in ms of wall time: the GPU's
+# clocks fall while the host prepares a workload, and a warmup of a few steps
+# times the ramp back up (the BK3 adjoint step: 185 us after 20 warmup steps,
+# 158 us after 200).
+WARM_MS = 100.0
+
+
+def time_steps(fn, steps, warmup, warm_ms=WARM_MS):
     """Average ms per step over `steps` calls of fn on the current stream,
-    bracketed by device synchronisation; HIP events on that stream."""
+    bracketed by device synchronisation; HIP events on that stream.  Before
+    them fn runs `warmup` times and then, until warm_ms of wall time have
+    passed, further untimed times (warm_ms = 0: exactly `warmup` times)."""
     import torch
+    start = time.perf_counter()
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
+    while (time.perf_counter() - start) * 1e3 < warm_ms:
+        for _ in range(max(1, warmup)):
+            fn()
+        torch.cuda.synchronize()
     s = torch.cuda.current_stream()
     t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     wall0 = time.perf_counter()
@@ -645,7 +659,7 @@ def run_only(a):
             vo = w["v"] if form != "bf16s" else torch.empty_like(u32)
             fn = lambda: ops.bk5(w["p"], w["D"], G32, u32, out=vo, form=form)
             nbytes = (32 if form == "bf16s" else 36) * w["dofs"]
-    ms, _ = time_steps(fn, a.steps, a.warmup)
+    ms, _ = time_steps(fn, a.steps, a.warmup, warm_ms=0)   # exactly W warmup steps: fixed dispatch counts
     print(json.dumps({"only": a.only, "ms_per_step": round(ms, 5), "GB/s": round(nbytes / (ms * 1e-3) * 1e-9, 1),
                       "bytes_per_step": nbytes}), flush=True)
 
@@ -665,7 +679,7 @@ def main():
 
     roofs = probe_roofs()
     w5 = bk5_workload()
-    ms5, wall5 = time_steps(w5["fn"], a.steps, a.warmup)
+    ms5, wall5 = time_steps(w5["fn"], a.steps, a.warmup, warm_ms=0)   # the headline: exactly W warmup steps
     gdofs5 = w5["dofs"] / (ms5 * 1e-3) * 1e-9
     gbs5 = w5["bytes"] / (ms5 * 1e-3) * 1e-9
 

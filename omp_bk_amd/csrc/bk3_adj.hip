@@ -19,11 +19,14 @@
 // per XCD are in flight and their factors (28.7 KB per element at p = 6 in
 // FP64) overflow the 4 MB L2 between the two reads, so the second read is
 // largely served from the MALL (rocprofv3: 1.85x the algorithmic read
-// bytes).  Keeping the first read's factors in registers (56 values per
-// lane) needs 256 VGPRs and one wave per SIMD, which measured 10-17%
-// slower; what is kept is that every other stream -- u, b (w), the
-// outputs and the second, last read of G and kappa -- is non-temporal, so
-// that the first read's lines are the ones the L2 holds (p = 6, E = 16384:
+// bytes).  It costs little time: served from a resident element instead,
+// the kernel ran 1% faster, and with both reads from a resident element
+// (no G traffic at all) 20% faster -- the kernel is bound by its
+// instruction stream and memory latency at three waves per SIMD, not by
+// HBM.  Keeping the first read's factors in registers (56 values per lane)
+// needs 256 VGPRs and one wave per SIMD, which measured 10-17% slower; what
+// is kept is that every other stream -- u, b (w), the outputs and the
+// second, last read of G and kappa -- is non-temporal (p = 6, E = 16384:
 // 169 -> 156 us in FP64, 128 -> 119 us with FP32 storage, same bits).  The
 // phases are those of the transposing BK3 mapping (bk3_phases.hpp).
 //
@@ -50,6 +53,12 @@ enum AdjMode { kLoss = 0, kVjp = 1, kApply = 2 };
 // divisors are the fastest measured on MI355X (the adjoint at 5.6 M DoF, p =
 // 6: one element per workgroup, 31.4 GDoF/s against 25.2 with four).
 constexpr int adj_div(int p) { return (p == 4 || p == 7) ? 1 : (p == 8 ? 2 : 4); }
+
+// Waves per SIMD the FP64-arithmetic kernels are held to (the occupancy
+// their registers give without spilling; unbounded, the unmasked loads of
+// p = 6 take 174 VGPRs and two waves).  The float-float kernels are left
+// unbounded.
+constexpr int adj_waves(int nm, bool ff) { return ff ? 1 : (nm <= 2 ? 5 : (nm <= 4 ? 4 : (nm <= 7 ? 3 : 2))); }
 
 template <int NM, int NQ>
 struct AdjShape {
@@ -120,7 +129,8 @@ struct AdjTypes<true> {
 // EO: Bm and Dm point at the even-odd forms of B and D (bkx_eo.hpp).
 // FF: float-float arithmetic (TS float); B and D are split into an LDS copy.
 template <int NM, int NQ, int MODE, typename TS, bool EO, bool FF = false>
-__global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
+__global__ __launch_bounds__((AdjShape<NM, NQ>::threads))
+__attribute__((amdgpu_waves_per_eu(adj_waves(NM, FF)))) void bk3_adj_kernel(
     size_t nelmt, const double* __restrict__ Bg, const double* __restrict__ Dg, const TS* __restrict__ G,
     const TS* __restrict__ kappa, const TS* __restrict__ u, const TS* __restrict__ bw,
     double* __restrict__ out_u, double* __restrict__ out_k, double* __restrict__ loss)
@@ -159,9 +169,12 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
 
     // Loads are unconditional, from clamped addresses inside a real element
     // (element 0 for idle lanes, column 0 for lanes outside the modal plane),
-    // and masked after the conversion: a load whose converted value is
-    // needed when a conditional block ends is waited for there, which for
-    // FP32 storage serialised every load of the kernel.
+    // and their values are not masked: an idle slot computes on element 0's
+    // data in its own LDS boxes and stores nothing, and a lane outside the
+    // modal plane holds modal values that no phase reads (r, whose squares
+    // are summed, is masked where it is formed).  A conditional load would
+    // be waited for where its block ends, and a select per loaded value cost
+    // 10-18% of the kernel's time (two selects per FP64 value).
     const int lm = modal ? l : 0;
     auto modal_load = [&](const TS* base, L (&dst)[NM]) {
         TS raw[NM];
@@ -171,7 +184,7 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
         }
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            dst[i] = (active && modal) ? L(raw[i]) : L(0);
+            dst[i] = L(raw[i]);
         }
     };
 
@@ -197,9 +210,9 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
             L g[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                g[f] = active ? L(graw[f]) : L(0);
+                g[f] = L(graw[f]);
             }
-            const L kap = active ? L(kraw) : L(0);
+            const L kap = L(kraw);
             const int xx = E::X(p, el.lq, el.lq2);
             const V qs = el.B[xx];
             const V qt = el.C[xx];
@@ -292,9 +305,9 @@ __global__ __launch_bounds__((AdjShape<NM, NQ>::threads)) void bk3_adj_kernel(
             L g[6];
 #pragma unroll
             for (int f = 0; f < 6; ++f) {
-                g[f] = active ? L(graw[f]) : L(0);
+                g[f] = L(graw[f]);
             }
-            const L kap = active ? L(kraw) : L(0);
+            const L kap = L(kraw);
             const int xx = E::X(p, el.lq, el.lq2);
             const V qs = el.B[xx];
             const V qt = el.C[xx];
@@ -332,15 +345,30 @@ int launch_adj(size_t nelmt, const double* B, const double* D, const TS* G, cons
 }
 
 // deterministic sum of n doubles into *out: one workgroup, fixed order
+// (thread t adds x[t], x[t + T], x[t + 2T], ... in turn, then a tree over
+// the threads).  The loads of kSumUnroll consecutive terms are issued
+// together; the additions keep their order.
 constexpr int kSumThreads = 1024;
+constexpr int kSumUnroll = 16;
 
 __global__ __launch_bounds__(kSumThreads) void sum_kernel(const double* __restrict__ x, size_t n,
                                                           double* __restrict__ out)
 {
     __shared__ double part[kSumThreads];
     double s = 0.0;
-    for (size_t i = threadIdx.x; i < n; i += kSumThreads) {
-        s += x[i];
+    for (size_t i0 = threadIdx.x; i0 < n; i0 += size_t(kSumThreads) * kSumUnroll) {
+        double t[kSumUnroll];
+#pragma unroll
+        for (int k = 0; k < kSumUnroll; ++k) {
+            const size_t i = i0 + size_t(k) * kSumThreads;
+            t[k] = x[i < n ? i : i0];
+        }
+#pragma unroll
+        for (int k = 0; k < kSumUnroll; ++k) {
+            if (i0 + size_t(k) * kSumThreads < n) {
+                s += t[k];
+            }
+        }
     }
     part[threadIdx.x] = s;
     __syncthreads();
