@@ -79,6 +79,7 @@ def test_driver_rejects_unsupported_order():
 @pytest.mark.parametrize("kind,p", [(1, 4), (3, 7), (5, 7)])
 def test_variants_harness_every_variant(kind, p):
     text = _run([os.path.join(BIN, "BK%d_variants" % kind), str(p), "3001", "2"])
+    assert "host reference on 3001 elements" in text
     rows = study.parse_variant_lines(text)
     count = _lib.lib().bkx_tune_count(kind, 0)
     assert [r[0] for r in rows] == ["v%d" % v for v in range(count)]
@@ -87,9 +88,18 @@ def test_variants_harness_every_variant(kind, p):
         assert gdofs > 0 and gbs > gdofs
 
 
+def test_variants_harness_strided_reference():
+    # beyond 8192 elements every ceil(nelmt / 8192)-th element is checked
+    text = _run([os.path.join(BIN, "BK1_variants"), "2", "20000", "1"], env={"BK_KERNEL": "v0"})
+    assert "host reference on %d elements" % len(range(0, 20000, 3)) in text
+    [(name, _, _, relerr)] = study.parse_variant_lines(text)
+    assert name == "v0" and relerr != "FAIL" and float(relerr) <= 1e-13
+
+
 def test_variants_harness_selection():
     # BK_KERNEL names variants to run, a leading "-" leaves one out
-    # (include/bk_harness.h:20-40); variant 0 runs regardless as the comparison
+    # (include/bk_harness.h:20-40); every variant is compared with the
+    # harness's host FP64 reference
     text = _run([os.path.join(BIN, "BK5_variants"), "3", "500", "1"], env={"BK_KERNEL": "v2 v7 -v7"})
     assert [r[0] for r in study.parse_variant_lines(text)] == ["v2"]
     text = _run([os.path.join(BIN, "BK5_variants"), "3", "500", "1"], env={"BK_KERNEL": "-v0 -v1"})
