@@ -6,7 +6,8 @@
   data (cos basis, in = 3, JxW = 1 or G = 2; openmp/BK*.cpp) through the
   output norm against the oracle's.
 * BK1_ff's claim that all float-float kernels sum every output in the same
-  index order and give the same result bit for bit (README.md:223-225): the
+  index order and give the same result bit for bit (the reference's
+  README.md:223-225): the
   launch variants of each float-float form agree bitwise."""
 import numpy as np
 import pytest
