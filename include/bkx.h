@@ -253,6 +253,21 @@ int bkx_bk3_vjp_f32c(int p, size_t nelmt, const double* basis, const double* dba
 /* *out = sum of x[0..n-1] in a fixed order (one workgroup; deterministic). */
 int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream);
 
+/* bkx_bk3_adjoint_f64 followed by bkx_sum_f64 of its per-element losses, with
+ * nothing allocated: L (one double) = 1/2 |A(kappa) u - b|^2, and dldu and
+ * dldkappa as bkx_bk3_adjoint_f64, the same bits as that pair of calls.  No
+ * counterpart in the reference: it is for callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops, harnesses that forbid
+ * hipMalloc).  The per-element losses live in a library-owned device array of
+ * BKX_ADJOINT_LOSS_MAX doubles (8 MiB) on each GPU the library runs on, held
+ * until the process ends; nelmt above that limit returns BKX_ERROR_SIZE.
+ * Calls that overlap on one GPU share the array and race.  The stream's
+ * device (the current device for the NULL stream) is the GPU used. */
+#define BKX_ADJOINT_LOSS_MAX ((size_t)1 << 20)
+int bkx_bk3_adjoint_loss_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                             const double* kappa, const double* u, const double* b, double* dldu, double* dldkappa,
+                             double* L, bkx_stream_t stream);
+
 /* ---------------------------------------------------------------------- */
 /* Geometric factors from element nodal coordinates (the step before the    */
 /* operators; no reference counterpart: the reference's operators take     */

@@ -43,6 +43,7 @@ _SIGNATURES = {
     "bkx_bk3_adjoint_f32c": [_c_int, _c_size] + [_vp] * 10,
     "bkx_bk3_vjp_f32c": [_c_int, _c_size] + [_vp] * 9,
     "bkx_sum_f64": [_vp, _c_size, _vp, _vp],
+    "bkx_bk3_adjoint_loss_f64": [_c_int, _c_size] + [_vp] * 10,
     "bkx_bk5_make_G_f64": [_c_int, _c_size, _vp, _c_int, _c_int, _vp, _vp],
     "bkx_bk5_make_G_f32": [_c_int, _c_size, _vp, _c_int, _c_int, _vp, _vp],
     "bkx_bk5_interleave_G_f64": [_c_int, _c_size, _c_int, _c_int, _vp, _vp, _vp],

@@ -39,6 +39,8 @@
 // contraction, flux and product a Dot2 compensated sum or TwoProd of
 // bkx_ff.hpp, the loaded floats exact).  dL/du, dL/dkappa and the loss are
 // FP64 in every form (the float-float ones joined with one add).
+#include <mutex>
+
 #include "bk3_phases.hpp"
 #include "bkx_launch.hpp"
 
@@ -497,6 +499,81 @@ int sum_f64(const double* x, size_t n, double* out, hipStream_t s)
 {
     hipLaunchKernelGGL(sum_kernel, dim3(1), dim3(kSumThreads), 0, s, x, n, out);
     return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// The loss of the FP64 adjoint with nothing allocated: the per-element losses
+// that bk3_adjoint_f64 writes to a caller's array go to `scratch`, a
+// library-owned array of kLossScratchMax doubles (8 MiB) on every GPU the
+// library runs on, held until the process ends; then sum_kernel adds them
+// into the caller's L.  Every entry the sum reads was written by the adjoint
+// kernel of the same call, so nothing is cleared.  In host code `scratch`
+// names a host-side stand-in, not a device address: kernels are given the
+// address hipGetSymbolAddress reports for the device, resolved once per device
+// with that device current.  Calls that overlap on one GPU share the array
+// and race.
+constexpr size_t kLossScratchMax = BKX_ADJOINT_LOSS_MAX;
+static __device__ __align__(256) double scratch[kLossScratchMax];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_by_device[kMaxDevices] = {};
+
+// The device address of `scratch` on device dev; nullptr if it cannot be had.
+double* loss_scratch(int dev)
+{
+    if (dev < 0 || dev >= kMaxDevices) {
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    if (!scratch_by_device[dev]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess) {
+            return nullptr;
+        }
+        if (cur != dev && hipSetDevice(dev) != hipSuccess) {
+            return nullptr;
+        }
+        void* ptr = nullptr;
+        const hipError_t err = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (cur != dev) {
+            (void)hipSetDevice(cur);
+        }
+        if (err != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        scratch_by_device[dev] = static_cast<double*>(ptr);
+    }
+    return scratch_by_device[dev];
+}
+
+}  // namespace
+
+int bk3_adjoint_loss_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
+                         const double* u, const double* b, double* dldu, double* dldk, double* L, hipStream_t s)
+{
+    if (nelmt > kLossScratchMax) {
+        return BKX_ERROR_SIZE;
+    }
+    int dev = -1;
+    if ((s ? hipStreamGetDevice(s, &dev) : hipGetDevice(&dev)) != hipSuccess) {
+        (void)hipGetLastError();
+        return BKX_ERROR_DEVICE;
+    }
+    double* part = loss_scratch(dev);
+    if (!part) {
+        return BKX_ERROR_DEVICE;
+    }
+    if (nelmt > 0) {
+        const int st = bk3_adjoint(p, nelmt, B, D, G, kappa, u, b, dldu, dldk, part, s);
+        if (st != BKX_SUCCESS) {
+            return st;
+        }
+    }
+    return sum_f64(part, nelmt, L, s);   // L = 0 for no elements
 }
 
 }  // namespace bkx

@@ -38,6 +38,8 @@ int bk3_adjoint_f32c(int p, size_t nelmt, const double* B, const double* D, cons
 int bk3_vjp_f32c(int p, size_t nelmt, const double* B, const double* D, const float* G, const float* kappa,
                  const float* u, const float* w, double* gu, double* gk, hipStream_t s);
 int sum_f64(const double* x, size_t n, double* out, hipStream_t s);
+int bk3_adjoint_loss_f64(int p, size_t nelmt, const double* B, const double* D, const double* G, const double* kappa,
+                         const double* u, const double* b, double* dldu, double* dldk, double* L, hipStream_t s);
 int geo_bk3_f64(int p, size_t nelmt, const double* B, const double* D, const double* wq, const double* coords,
                 double* G, double* JxW, hipStream_t s);
 int geo_bk5_f64(int p, size_t nelmt, const double* D, const double* wq, const double* coords, double* G,
@@ -693,6 +695,22 @@ int bkx_geo_bk5_f64(int p, size_t nelmt, const double* dbasis, const double* qwe
         return BKX_ERROR_NULL;
     }
     return bkx::geo_bk5_f64(p, nelmt, dbasis, qweights, coords, G, as_stream(stream));
+}
+
+int bkx_bk3_adjoint_loss_f64(int p, size_t nelmt, const double* basis, const double* dbasis, const double* G,
+                             const double* kappa, const double* u, const double* b, double* dldu, double* dldkappa,
+                             double* L, bkx_stream_t stream)
+{
+    if (!order_ok(3, p)) {
+        return BKX_ERROR_ORDER;
+    }
+    if (!L || (nelmt > 0 && (!basis || !dbasis || !G || !kappa || !u || !b || !dldu || !dldkappa))) {
+        return BKX_ERROR_NULL;
+    }
+    if (nelmt > BKX_ADJOINT_LOSS_MAX) {
+        return BKX_ERROR_SIZE;
+    }
+    return bkx::bk3_adjoint_loss_f64(p, nelmt, basis, dbasis, G, kappa, u, b, dldu, dldkappa, L, as_stream(stream));
 }
 
 int bkx_sum_f64(const double* x, size_t n, double* out, bkx_stream_t stream)

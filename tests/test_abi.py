@@ -50,6 +50,12 @@ def test_error_paths_without_device():
     assert [L.bkx_max_order(k) for k in (1, 3, 5)] == [8, 8, 7]
     assert L.bkx_max_order(2) == -1
     assert L.bkx_status_string(1) == b"unsupported polynomial order p"
+    # the allocation-free adjoint: order, NULL and its element limit are
+    # checked on the host (the fake pointers are never dereferenced)
+    fake = [ctypes.c_void_p(256)] * 8
+    assert L.bkx_bk3_adjoint_loss_f64(9, 2, *fake, fake[0], None) == 1
+    assert L.bkx_bk3_adjoint_loss_f64(3, 2, *fake, None, None) == 2
+    assert L.bkx_bk3_adjoint_loss_f64(3, (1 << 20) + 1, *fake, fake[0], None) == 4
     # geometric factors and element restriction: checked before device work
     assert L.bkx_geo_bk3_f64(9, 10, None, None, None, None, None, None, None) == 1
     assert L.bkx_geo_bk5_f64(8, 10, None, None, None, None, None) == 1
