@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--only", choices=["bk5", "bk5_f32w", "bk5_f32c", "bk5_bf16s", "bk5_oz", "bk3", "copy", "mixed",
                                        "adjoint_f64", "adjoint_f32w", "adjoint_f32c", "adjoint_f64_eo",
-                                       "adjoint_f32w_eo"],
+                                       "adjoint_f32w_eo", "fp64rate"],
                     default=None,
                     help="run one workload only (for profiling) and print its timing")
     return ap.parse_args()
@@ -508,6 +508,24 @@ def probe_roofs(nbytes=2 << 30, steps=20):
     return out
 
 
+def fp64_rates(iters=4000, steps=5):
+    """Measured FP64 issue rates (TFLOP/s) of the vector FMA, of the FP64
+    matrix-core product v_mfma_f64_16x16x4_f64, and of both in one
+    instruction stream (bkx_probe_fp64_rate), four wavefronts per SIMD."""
+    import torch
+    from omp_bk_amd import _lib
+    n = 1024 * 256
+    out = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    st = lambda: torch.cuda.current_stream().cuda_stream
+    res = {}
+    for kind, name, flops in ((0, "valu_fma", 8192), (1, "mfma_16x16x4", 8192), (2, "both", 16384)):
+        ms, _ = time_steps(lambda: _lib.call("bkx_probe_fp64_rate", kind, iters, out.data_ptr(), n, st()), steps, 2,
+                           warm_ms=20.0)
+        res[name] = round(n / 64 * iters * flops / (ms * 1e-3) * 1e-12, 2)
+    del out
+    return res
+
+
 def cpu_baseline(budget_s=3.0):
     """The reference's own CPU operators, SumFactorization<double, nq> for
     BK1 p = 3 (E = 4096), BK3 p = 5 (E = 32768) and BK5 p = 7 (E = 65536),
@@ -627,6 +645,9 @@ def run_only(a):
     if a.only == "mixed":
         print(json.dumps({"only": "mixed", **mixed_layouts(a.steps, a.warmup)}), flush=True)
         return
+    if a.only == "fp64rate":
+        print(json.dumps({"only": "fp64rate", "TFLOP/s": fp64_rates(steps=a.steps)}), flush=True)
+        return
     if a.only == "copy":
         n = (2 << 30) // 8
         x = torch.ones(n, dtype=torch.float64, device="cuda:0")
@@ -715,6 +736,9 @@ def main():
     lines.update(adjoint_config(a.steps, a.warmup, forms=("f32c",),
                                 ms64=lines["bk3_adjoint_p6_f64"]["ms_per_step"]))
 
+    # last on the GPU: arithmetic at full rate may lower the clocks of what
+    # follows it
+    rates = fp64_rates()
     cpu = None if a.no_cpu else cpu_baseline()
 
     line = {
@@ -736,6 +760,7 @@ def main():
                      "frac": round(gbs5 / HBM_PEAK_GBS, 4), "traffic": traffic("bk5_p7_f64")},
         "cpu_baseline": cpu,
         "measured_roofs_GBs": roofs,
+        "measured_fp64_TFLOPs": rates,
         "frac_of_read7": round(gbs5 / roofs["read7"], 4),
         "wall_ms_per_step": round(wall5, 5),
         **forms,

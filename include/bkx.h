@@ -432,7 +432,7 @@ int bkx_graph_launch(bkx_graph* graph, bkx_stream_t stream);
 int bkx_graph_destroy(bkx_graph* graph);
 
 /* ---------------------------------------------------------------------- */
-/* Bandwidth probes (the roofs the operators are measured against)         */
+/* Probes (the roofs the operators are measured against)                   */
 /* ---------------------------------------------------------------------- */
 
 /* c = a for n doubles (n even).  HIP counterpart of the copy roof of
@@ -444,6 +444,16 @@ int bkx_probe_copy(const double* a, double* c, size_t n, bkx_stream_t stream);
  * written.  R = 9 is the reference's read9 roof (roofline/stream_omp.cpp:
  * 101-117); R = 7 matches BK5 in FP64 (56 B read, 8 B written per DoF). */
 int bkx_probe_read_mix(int reads_per_write, const double* a, double* c, size_t n_out, bkx_stream_t stream);
+
+/* FP64 arithmetic issue rates (no reference counterpart: the reference's
+ * roofs are bandwidth only).  n / 256 workgroups of 256 threads each run
+ * `iters` iterations of, per wavefront,
+ *   kind 0: 64 v_fma_f64 (128 flops each),
+ *   kind 1: 4 v_mfma_f64_16x16x4_f64 (2048 flops each),
+ *   kind 2: both, in one instruction stream,
+ * and write one value per thread to out (n doubles, a multiple of 256).
+ * Flops per launch: n / 64 * iters * (8192 kind 0, 8192 kind 1, 16384 kind 2). */
+int bkx_probe_fp64_rate(int kind, int iters, double* out, size_t n, bkx_stream_t stream);
 
 #ifdef __cplusplus
 }

@@ -86,15 +86,22 @@ __attribute__((amdgpu_waves_per_eu(MINW))) void bk3p_kernel(
         return;
     }
     constexpr size_t qper = 6 * S::nq3;
-    // element of this thread in group grp (element 0 where there is none)
+    // element of this thread in group grp: threads past the slots take slot
+    // 0's (the box they read), slots past the count element 0
     auto elem = [&](size_t grp) {
-        const size_t e = grp * S::epb + slot;
-        return slot < S::epb && e < nelmt ? e : size_t(0);
+        const size_t e = grp * S::epb + (slot < S::epb ? slot : 0);
+        return e < nelmt ? e : size_t(0);
     };
     size_t ee = elem(g);
     double xin[NM];
     double qin[bk13t::bk13t_nqv(true, GPF, NQ)];
     bk13t::bk13t_load<NM, NQ, true, DIV, GPF, BKX_FORM_F64, NT, true>(Q + ee * qper, u + ee * S::nm3, slot, l, xin, qin);
+    // The first element's loads complete before the loop (once per wave):
+    // the path into the loop from here then has nothing in flight, and the
+    // compiler's waits at the top of the loop are those of the back edge,
+    // which counts the pass's stores (always NM, issued after its loads of
+    // the next element) instead of waiting for them.
+    __builtin_amdgcn_s_waitcnt(0x0f70);   // vmcnt(0)
     for (;;) {
         const size_t e = g * S::epb + slot;
         const bool active = slot < S::epb && e < nelmt;

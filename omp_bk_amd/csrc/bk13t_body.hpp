@@ -800,7 +800,27 @@ __device__ __forceinline__ void bk13t_phases(const MT* __restrict__ B, const MT*
     __syncthreads();
 
     // ---- P9: lane (j,k), contract p -> i, store --------------------------
-    if (active && l < S::nm2) {
+    if constexpr (PIPE) {
+        // Every lane stores, so that every pass issues the same NM stores
+        // after its loads of the next element and the wait for those loads
+        // at the top of the loop counts the stores instead of waiting for
+        // them.  A lane outside the modal plane stores the column of plane
+        // lane l mod nm^2 of its own box: the same bits as that lane, to the
+        // same addresses.  Lanes past the element count hold the clamped
+        // element's data and ve, and lanes past the slots slot 0's (the
+        // caller points ve there), so theirs are that element's own values.
+        const int lp = l < S::nm2 ? l : l % S::nm2;
+        V z[NQ], o[NM];
+#pragma unroll
+        for (int p = 0; p < NQ; ++p) {
+            z[p] = bB[X(p, lp / NM, lp % NM)];
+        }
+        contract_rows<NQ, SB>(mat_b<12, EO, NM, NQ>(B, dep), z, o);
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            store_v<(NT & 1) != 0>(ve + i * S::nm2 + lp, o[i]);
+        }
+    } else if (active && l < S::nm2) {
         V z[NQ], o[NM];
 #pragma unroll
         for (int p = 0; p < NQ; ++p) {

@@ -1,5 +1,6 @@
 // probe.hip -- bandwidth probes for the roofline of the streaming operators,
-// the HIP counterparts of the reference's STREAM-like roofs
+// and an FP64 issue-rate probe (vector FMAs against FP64 matrix-core
+// products).  The bandwidth probes are the HIP counterparts of the reference's STREAM-like roofs
 // (roofline/stream_omp.cpp:86-117): a copy, and a read-heavy mix that reads
 // R blocks for every block it writes (R = 9 is the reference's read9, BK3's
 // share of reads; R = 7 is BK5's 56 B read : 8 B written per DoF in FP64).
@@ -61,10 +62,100 @@ __global__ __launch_bounds__(kProbeThreads) void read_mix_kernel(const f64x2* __
     }
 }
 
+// FP64 arithmetic issue rates, the compute side of the roofline and the
+// facts behind keeping the BK3 contractions on the vector ALU:
+//   KIND 0: kRateFma vector FMAs per iteration (v_fma_f64, 128 flops per
+//           wavefront instruction), in kFmaChains independent chains;
+//   KIND 1: kRateMfma matrix-core products per iteration
+//           (v_mfma_f64_16x16x4_f64, 2048 flops each), in as many chains;
+//   KIND 2: both in one instruction stream (whether the two pipes overlap).
+// Every lane writes one sum of its chains, so nothing is dead code.
+constexpr int kRateThreads = 256;
+constexpr int kFmaChains = 16;
+constexpr int kRateFma = 64;    // 4 per chain
+constexpr int kRateMfma = 4;
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <int KIND>
+__global__ __launch_bounds__(kRateThreads) void fp64_rate_kernel(int iters, double* __restrict__ out)
+{
+    const double a = 1.0 + 1e-6 * threadIdx.x;
+    const double b = 1.0 - 1e-6 * (threadIdx.x & 7);
+    double f[kFmaChains];
+#pragma unroll
+    for (int k = 0; k < kFmaChains; ++k) {
+        f[k] = 1e-3 * (k + 1);
+    }
+    f64x4 c[kRateMfma];
+#pragma unroll
+    for (int k = 0; k < kRateMfma; ++k) {
+        c[k] = f64x4{0.0, 0.0, 0.0, 1e-3 * k};
+    }
+    for (int it = 0; it < iters; ++it) {
+        if constexpr (KIND != 1) {
+#pragma unroll
+            for (int r = 0; r < kRateFma / kFmaChains; ++r) {
+#pragma unroll
+                for (int k = 0; k < kFmaChains; ++k) {
+                    f[k] = __builtin_fma(f[k], b, a);
+                }
+            }
+        }
+        if constexpr (KIND != 0) {
+#pragma unroll
+            for (int k = 0; k < kRateMfma; ++k) {
+                c[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c[k], 0, 0, 0);
+            }
+        }
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < kFmaChains; ++k) {
+        s += f[k];
+    }
+#pragma unroll
+    for (int k = 0; k < kRateMfma; ++k) {
+        s += c[k][0] + c[k][1] + c[k][2] + c[k][3];
+    }
+    out[size_t(blockIdx.x) * kRateThreads + threadIdx.x] = s;
+}
+
 }  // namespace
 }  // namespace bkx
 
 extern "C" {
+
+int bkx_probe_fp64_rate(int kind, int iters, double* out, size_t n, bkx_stream_t stream)
+{
+    if (n == 0) {
+        return BKX_SUCCESS;
+    }
+    if (!out) {
+        return BKX_ERROR_NULL;
+    }
+    if (n % bkx::kRateThreads || iters < 0) {
+        return BKX_ERROR_ARG;
+    }
+    const unsigned grid = bkx::grid_for(n, bkx::kRateThreads);
+    if (grid == 0) {
+        return BKX_ERROR_SIZE;
+    }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    switch (kind) {
+    case 0:
+        hipLaunchKernelGGL(bkx::fp64_rate_kernel<0>, dim3(grid), dim3(bkx::kRateThreads), 0, s, iters, out);
+        break;
+    case 1:
+        hipLaunchKernelGGL(bkx::fp64_rate_kernel<1>, dim3(grid), dim3(bkx::kRateThreads), 0, s, iters, out);
+        break;
+    case 2:
+        hipLaunchKernelGGL(bkx::fp64_rate_kernel<2>, dim3(grid), dim3(bkx::kRateThreads), 0, s, iters, out);
+        break;
+    default:
+        return BKX_ERROR_ARG;
+    }
+    return bkx::launch_status();
+}
 
 int bkx_probe_copy(const double* a, double* c, size_t n, bkx_stream_t stream)
 {
