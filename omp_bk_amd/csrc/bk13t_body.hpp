@@ -415,9 +415,11 @@ constexpr int bk13t_nqv(bool bk3, int gpf, int nq) { return bk3 ? (gpf > 0 ? gpf
 // element (Qn), and after P5 the lane's next u column (un) replaces xin, so
 // that the next pass starts with its first loads already in flight.  In that
 // mode the loads of u are unconditional (lanes outside the element read a
-// valid column, clamped) and the prologue loads G before u, as the phases
-// do, so that the memory operations in flight at the top of the loop are the
-// same on both paths into it and the compiler's wait counts stay exact.
+// valid column, clamped), the prologue loads G before u, as the phases do,
+// and every lane stores in P9 (the kernel waits for its prologue's loads
+// before the loop): each pass then ends with the same loads and stores in
+// flight, and the compiler's wait for the next element's loads at the top of
+// the loop counts the stores instead of waiting for them.
 // With EO (FP64 arithmetic) B and D point at their even-odd forms (EoMat),
 // and every contraction runs in that form; the phases are unchanged.
 template <int NM, int NQ, bool BK3, int DIV, int GPF, int FORM, int NT, bool PIPE>
