@@ -41,10 +41,14 @@ struct MixedTile {
 };
 
 constexpr int kMixedThreads = 256;
-// every order's body: two LDS boxes per element, two rows of G loaded ahead
-// (the fastest single-order BK3 form at most orders)
+// every order's body: two LDS boxes per element, and three rows of G loaded
+// ahead (the fastest single-order BK3 form at most orders); two in the
+// persistent loop, where a third takes its registers past four waves per
+// SIMD (134 VGPRs)
 constexpr int kMixedBoxes = 2;
-constexpr int kMixedGpf = 2;
+constexpr int mixed_gpf(bool loop) { return loop ? 2 : 3; }
+// u loaded and v stored with the non-temporal hint (each is touched once)
+constexpr int kMixedNt = 1;
 
 template <int P>
 constexpr int smem_of()
@@ -84,14 +88,14 @@ __device__ __forceinline__ void run_tile(int tid, const MixedTile& t, const int6
         // one round: the tile holds at most epb elements
         const bool active = slot < S::epb && slot < t.count;
         const int e = t.first + (active ? slot : 0);
-        bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf, BKX_FORM_F64, double, 0, EO>(
+        bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, mixed_gpf(LOOP), BKX_FORM_F64, double, kMixedNt, EO>(
             B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l, smem);
     } else {
         for (int r0 = 0; r0 < t.count; r0 += S::epb) {
             const int idx = r0 + slot;
             const bool active = slot < S::epb && idx < t.count;
             const int e = t.first + (active ? idx : 0);
-            bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, kMixedGpf, BKX_FORM_F64, double, 0, EO>(
+            bk13t::bk13t_body<NM, NQ, true, DIV, false, kMixedBoxes, mixed_gpf(LOOP), BKX_FORM_F64, double, kMixedNt, EO>(
                 B, D, G + goff[e], u + uoff[e], v + uoff[e], active, slot, l, smem, t.first + r0);
             __syncthreads();
         }
