@@ -108,15 +108,30 @@ __global__ __launch_bounds__((GeoShape<NM, NQ, EPB>::threads)) void geo_gauss_ke
     // the LDS then holds only the gradient's three boxes, so that several
     // workgroups share a CU
     double Jc[3][3][NQ];
+    // the lane's column of coordinate c, loaded unconditionally (column 0
+    // outside the modal plane, whose values no phase reads), one coordinate
+    // ahead of its gradient so that the next column's loads are in flight
+    // during the current gradient
+    const int lm = modal ? l : 0;
+    double xn[NM], x0n;
+    auto load_col = [&](int c) {
+        const double* xc = coords + (ee * 3 + c) * E::nm3;
+        x0n = xc[0];
+#pragma unroll
+        for (int i = 0; i < NM; ++i) {
+            xn[i] = xc[lm + i * E::nm2];
+        }
+    };
+    load_col(0);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
         double x[NM];
-        const double* xc = coords + (ee * 3 + c) * E::nm3;
-        const double x0 = xc[0];
-        const double* xe = xc + (modal ? l : 0);
 #pragma unroll
         for (int i = 0; i < NM; ++i) {
-            x[i] = modal ? xe[i * E::nm2] - x0 : 0.0;
+            x[i] = xn[i] - x0n;
+        }
+        if (c < 2) {
+            load_col(c + 1);
         }
         double qr[NQ];
         el.template grad<0>(Bm, Dm, x, qr);   // ends with the workgroup synchronised
