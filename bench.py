@@ -487,9 +487,9 @@ def reduced_forms(w, steps, warmup, ms64, forms=("f32w", "bf16s", "f32c")):
 
 
 def probe_roofs(nbytes=2 << 30, steps=20):
-    """Measured HBM roofs (GB/s, bytes read + written): a copy and the
+    """Measured HBM roofs (GB/s, bytes read + written): a copy, the
     read-heavy mixes R = 7 (BK5 FP64's 56 B read : 8 B written per DoF) and
-    R = 9 (the reference's read9)."""
+    R = 9 (the reference's read9), and a read alone (the read ceiling)."""
     import torch
     from omp_bk_amd import _lib
     n = nbytes // 8
@@ -504,6 +504,8 @@ def probe_roofs(nbytes=2 << 30, steps=20):
         ms, _ = time_steps(lambda: _lib.call("bkx_probe_read_mix", r, a.data_ptr(), c.data_ptr(), nout, st()),
                            steps, 3)
         out["read%d" % r] = round(8 * nout * (r + 1) / (ms * 1e-3) * 1e-9, 1)
+    ms, _ = time_steps(lambda: _lib.call("bkx_probe_read", a.data_ptr(), n, c.data_ptr(), st()), steps, 3)
+    out["read"] = round(nbytes / (ms * 1e-3) * 1e-9, 1)
     del a, c
     torch.cuda.empty_cache()
     return out
@@ -768,6 +770,7 @@ def main():
         "measured_roofs_GBs": roofs,
         "measured_fp64_TFLOPs": rates,
         "frac_of_read7": round(gbs5 / roofs["read7"], 4),
+        "frac_of_read": round(gbs5 / roofs["read"], 4),
         "wall_ms_per_step": round(wall5, 5),
         **forms,
         **lines,

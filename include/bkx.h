@@ -445,6 +445,12 @@ int bkx_probe_copy(const double* a, double* c, size_t n, bkx_stream_t stream);
  * 101-117); R = 7 matches BK5 in FP64 (56 B read, 8 B written per DoF). */
 int bkx_probe_read_mix(int reads_per_write, const double* a, double* c, size_t n_out, bkx_stream_t stream);
 
+/* Reads n doubles (n even) once, as 16-byte non-temporal loads, and writes
+ * c[0 .. ceil(n / 16384) * 8 - 1] (one 16-byte sum per wavefront of 4096
+ * values): the HBM read ceiling, which the read-heavy mixes above stay
+ * below.  No reference counterpart. */
+int bkx_probe_read(const double* a, size_t n, double* c, bkx_stream_t stream);
+
 /* FP64 arithmetic issue rates (no reference counterpart: the reference's
  * roofs are bandwidth only).  n / 256 workgroups of 256 threads each run
  * `iters` iterations of, per wavefront,

@@ -61,6 +61,7 @@ _SIGNATURES = {
     "bkx_probe_copy": [_vp, _vp, _c_size, _vp],
     "bkx_probe_read_mix": [_c_int, _vp, _vp, _c_size, _vp],
     "bkx_probe_fp64_rate": [_c_int, _c_int, _vp, _c_size, _vp],
+    "bkx_probe_read": [_vp, _c_size, _vp, _vp],
     "bkx_tune_count": [_c_int, _c_int],
     "bkx_tune_set": [_c_int, _c_int, _c_int, _c_int],
     "bkx_tune_get": [_c_int, _c_int, _c_int],

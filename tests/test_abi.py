@@ -61,6 +61,9 @@ def test_error_paths_without_device():
     assert L.bkx_probe_fp64_rate(0, 1, fake[0], 100, None) == 5
     assert L.bkx_probe_fp64_rate(0, 1, None, 256, None) == 2
     assert L.bkx_probe_fp64_rate(0, 1, None, 0, None) == 0
+    assert L.bkx_probe_read(fake[0], 3, fake[0], None) == 5
+    assert L.bkx_probe_read(None, 4, fake[0], None) == 2
+    assert L.bkx_probe_read(None, 0, None, None) == 0
     # geometric factors and element restriction: checked before device work
     assert L.bkx_geo_bk3_f64(9, 10, None, None, None, None, None, None, None) == 1
     assert L.bkx_geo_bk5_f64(8, 10, None, None, None, None, None) == 1

@@ -35,6 +35,23 @@ def test_copy_and_read_mix():
         assert np.array_equal(got, want), r
 
 
+def test_read_probe():
+    n = 16384 * 5 + 2   # a partial last workgroup
+    a = torch.rand(n, dtype=torch.float64, device=DEV)
+    nc = -(-n // 16384) * 8
+    c = torch.full((nc + 8,), float("nan"), dtype=torch.float64, device=DEV)
+    _lib.call("bkx_probe_read", a.data_ptr(), n, c.data_ptr(), _st())
+    torch.cuda.synchronize()
+    got = c.cpu().numpy()
+    assert np.isnan(got[nc:]).all()          # nothing written past the documented extent
+    assert np.isfinite(got[:nc]).all()
+    ah = a.cpu().numpy()
+    # lane pair t of wavefront w of workgroup g sums its 32 vectors; the
+    # wavefront's sums are added by a butterfly: check the totals
+    assert abs(got[:nc].sum() - ah.sum()) <= 1e-9 * ah.sum()
+    assert abs(got[:nc:2].sum() - ah[0::2].sum()) <= 1e-9 * ah.sum()
+
+
 def test_fp64_rate_probe():
     n, iters = 512, 3
     out = torch.full((n,), float("nan"), dtype=torch.float64, device=DEV)
