@@ -512,16 +512,18 @@ def probe_roofs(nbytes=2 << 30, steps=20):
 
 
 def fp64_rates(iters=4000, steps=5):
-    """Measured FP64 issue rates (TFLOP/s) of the vector FMA, of the FP64
-    matrix-core product v_mfma_f64_16x16x4_f64, and of both in one
-    instruction stream (bkx_probe_fp64_rate), four wavefronts per SIMD."""
+    """Measured issue rates (TFLOP/s) of the FP64 vector FMA, of the FP64
+    matrix-core product v_mfma_f64_16x16x4_f64, of both in one instruction
+    stream, and of the non-packed FP32 vector FMA (bkx_probe_fp64_rate),
+    four wavefronts per SIMD."""
     import torch
     from omp_bk_amd import _lib
     n = 1024 * 256
     out = torch.empty(n, dtype=torch.float64, device="cuda:0")
     st = lambda: torch.cuda.current_stream().cuda_stream
     res = {}
-    for kind, name, flops in ((0, "valu_fma", 8192), (1, "mfma_16x16x4", 8192), (2, "both", 16384)):
+    for kind, name, flops in ((0, "valu_fma", 8192), (1, "mfma_16x16x4", 8192), (2, "both", 16384),
+                              (3, "valu_fma_f32", 8192)):
         ms, _ = time_steps(lambda: _lib.call("bkx_probe_fp64_rate", kind, iters, out.data_ptr(), n, st()), steps, 2,
                            warm_ms=20.0)
         res[name] = round(n / 64 * iters * flops / (ms * 1e-3) * 1e-12, 2)

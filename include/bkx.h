@@ -451,14 +451,15 @@ int bkx_probe_read_mix(int reads_per_write, const double* a, double* c, size_t n
  * below.  No reference counterpart. */
 int bkx_probe_read(const double* a, size_t n, double* c, bkx_stream_t stream);
 
-/* FP64 arithmetic issue rates (no reference counterpart: the reference's
+/* FP64 (and FP32) arithmetic issue rates (no reference counterpart: the reference's
  * roofs are bandwidth only).  n / 256 workgroups of 256 threads each run
  * `iters` iterations of, per wavefront,
  *   kind 0: 64 v_fma_f64 (128 flops each),
  *   kind 1: 4 v_mfma_f64_16x16x4_f64 (2048 flops each),
  *   kind 2: both, in one instruction stream,
+ *   kind 3: 64 non-packed v_fma_f32 (128 flops each),
  * and write one value per thread to out (n doubles, a multiple of 256).
- * Flops per launch: n / 64 * iters * (8192 kind 0, 8192 kind 1, 16384 kind 2). */
+ * Flops per launch: n / 64 * iters * (8192 kinds 0, 1 and 3, 16384 kind 2). */
 int bkx_probe_fp64_rate(int kind, int iters, double* out, size_t n, bkx_stream_t stream);
 
 #ifdef __cplusplus

@@ -102,7 +102,10 @@ __global__ __launch_bounds__(kProbeThreads) void read_kernel(const f64x2* __rest
 //           wavefront instruction), in kFmaChains independent chains;
 //   KIND 1: kRateMfma matrix-core products per iteration
 //           (v_mfma_f64_16x16x4_f64, 2048 flops each), in as many chains;
-//   KIND 2: both in one instruction stream (whether the two pipes overlap).
+//   KIND 2: both in one instruction stream (whether the two pipes overlap);
+//   KIND 3: kRateFma non-packed FP32 FMAs per iteration (v_fma_f32, 128
+//           flops per wavefront instruction; packed FP32 is disabled for
+//           every kernel here), the rate float-float arithmetic runs at.
 // Every lane writes one sum of its chains, so nothing is dead code.
 constexpr int kRateThreads = 256;
 constexpr int kFmaChains = 16;
@@ -125,8 +128,23 @@ __global__ __launch_bounds__(kRateThreads) void fp64_rate_kernel(int iters, doub
     for (int k = 0; k < kRateMfma; ++k) {
         c[k] = f64x4{0.0, 0.0, 0.0, 1e-3 * k};
     }
+    float h[kFmaChains];
+#pragma unroll
+    for (int k = 0; k < kFmaChains; ++k) {
+        h[k] = 1e-3f * float(k + 1);
+    }
+    const float af = float(a), bf = float(b);
     for (int it = 0; it < iters; ++it) {
-        if constexpr (KIND != 1) {
+        if constexpr (KIND == 3) {
+#pragma unroll
+            for (int r = 0; r < kRateFma / kFmaChains; ++r) {
+#pragma unroll
+                for (int k = 0; k < kFmaChains; ++k) {
+                    h[k] = __builtin_fmaf(h[k], bf, af);
+                }
+            }
+        }
+        if constexpr (KIND == 0 || KIND == 2) {
 #pragma unroll
             for (int r = 0; r < kRateFma / kFmaChains; ++r) {
 #pragma unroll
@@ -135,7 +153,7 @@ __global__ __launch_bounds__(kRateThreads) void fp64_rate_kernel(int iters, doub
                 }
             }
         }
-        if constexpr (KIND != 0) {
+        if constexpr (KIND == 1 || KIND == 2) {
 #pragma unroll
             for (int k = 0; k < kRateMfma; ++k) {
                 c[k] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c[k], 0, 0, 0);
@@ -150,6 +168,10 @@ __global__ __launch_bounds__(kRateThreads) void fp64_rate_kernel(int iters, doub
 #pragma unroll
     for (int k = 0; k < kRateMfma; ++k) {
         s += c[k][0] + c[k][1] + c[k][2] + c[k][3];
+    }
+#pragma unroll
+    for (int k = 0; k < kFmaChains; ++k) {
+        s += double(h[k]);
     }
     out[size_t(blockIdx.x) * kRateThreads + threadIdx.x] = s;
 }
@@ -206,6 +228,9 @@ int bkx_probe_fp64_rate(int kind, int iters, double* out, size_t n, bkx_stream_t
         break;
     case 2:
         hipLaunchKernelGGL(bkx::fp64_rate_kernel<2>, dim3(grid), dim3(bkx::kRateThreads), 0, s, iters, out);
+        break;
+    case 3:
+        hipLaunchKernelGGL(bkx::fp64_rate_kernel<3>, dim3(grid), dim3(bkx::kRateThreads), 0, s, iters, out);
         break;
     default:
         return BKX_ERROR_ARG;

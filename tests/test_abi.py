@@ -57,7 +57,7 @@ def test_error_paths_without_device():
     assert L.bkx_bk3_adjoint_loss_f64(3, 2, *fake, None, None) == 2
     assert L.bkx_bk3_adjoint_loss_f64(3, (1 << 20) + 1, *fake, fake[0], None) == 4
     # probes: argument checks
-    assert L.bkx_probe_fp64_rate(3, 1, fake[0], 256, None) == 5
+    assert L.bkx_probe_fp64_rate(4, 1, fake[0], 256, None) == 5
     assert L.bkx_probe_fp64_rate(0, 1, fake[0], 100, None) == 5
     assert L.bkx_probe_fp64_rate(0, 1, None, 256, None) == 2
     assert L.bkx_probe_fp64_rate(0, 1, None, 0, None) == 0

@@ -63,10 +63,12 @@ def test_fp64_rate_probe():
     f = [np.full(n, 1e-3 * (k + 1)) for k in range(16)]
     for _ in range(iters * 4):   # 64 FMAs per iteration over 16 chains
         f = [x * b + a for x in f]
-    want = sum(f) + sum(1e-3 * k for k in range(4))   # + the untouched MFMA accumulators
+    # + the untouched MFMA accumulators and FP32 chains
+    want = sum(f) + sum(1e-3 * k for k in range(4))
+    want = want + sum(float(np.float32(1e-3) * np.float32(k + 1)) for k in range(16))
     got = out.cpu().numpy()
     assert np.allclose(got, want, rtol=1e-13, atol=0)
-    for kind in (1, 2):
+    for kind in (1, 2, 3):
         out.fill_(float("nan"))
         _lib.call("bkx_probe_fp64_rate", kind, iters, out.data_ptr(), n, _st())
         torch.cuda.synchronize()
