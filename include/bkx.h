@@ -22,6 +22,10 @@
  * The plain FP64 entry points are the drop-in for the reference's
  * SumFactorization<double, nq> (the reference kernels return void and run
  * synchronously; the C++ shim in bkx_sumfact.hpp restores that).
+ *
+ * Outputs must not overlap inputs (no in-place apply): the kernels read
+ * their inputs while other workgroups write outputs, and some launch
+ * variants write an output value more than once (the same value).
  */
 #ifndef BKX_H
 #define BKX_H
