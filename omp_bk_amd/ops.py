@@ -39,6 +39,15 @@ def _check(name, t, numel, dtype=torch.float64):
         raise ValueError("%s has %d values, expected %d" % (name, t.numel(), numel))
 
 
+def _apart(out, u):
+    """Raises on an in-place apply (out at u's address): the kernels read u
+    while other workgroups write out.  Only that case is checked, to keep the
+    per-call host cost low (0.1 us); any other overlap of an output with an
+    input is as undefined as in the C ABI."""
+    if out.data_ptr() == u.data_ptr() and u.numel():
+        raise ValueError("out must not be u (no in-place apply)")
+
+
 def _stream(t):
     """The handle of the current torch stream on t's device."""
     idx = t.get_device()
@@ -107,6 +116,7 @@ def bk1(p, basis, JxW, u, out=None, form="f64"):
     if out is None:
         out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nm ** 3)
+    _apart(out, u)
     with _on(u):
         _lib.call(entry, p, ne, basis.data_ptr(), JxW.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
     return out
@@ -133,6 +143,7 @@ def bk3(p, basis, dbasis, G, u, out=None, form="f64"):
     if out is None:
         out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nm ** 3)
+    _apart(out, u)
     with _on(u):
         _lib.call(entry, p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(),
                   _stream(u))
@@ -181,6 +192,7 @@ def bk3_prepared(p, prepared, G, u, out=None, form="f64"):
     if out is None:
         out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nm ** 3)
+    _apart(out, u)
     with _on(u):
         _lib.call(entry, p, ne, prepared.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
     return out
@@ -221,6 +233,7 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
     if out is None:
         out = torch.empty(u.shape, dtype=odt, device=u.device)
     _check("out", out, ne * nq ** 3, odt)
+    _apart(out, u)
     with _on(u):
         _lib.call(entry, p, ne, dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
     return out
@@ -255,6 +268,7 @@ def bk3_kappa(p, basis, dbasis, G, kappa, u, out=None, form="f64"):
     if out is None:
         out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nm ** 3)
+    _apart(out, u)
     with _on(u):
         _lib.call("bkx_bk3_apply_kappa_" + form, p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(),
                   kappa.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))

@@ -131,6 +131,18 @@ def test_empty_and_errors():
         ops.bk5(8, dev(np.zeros(81)), z, z)
     with pytest.raises(TypeError):
         ops.bk5(3, D, z.float(), z)
+    # no in-place apply: the kernels read u while other workgroups write out
+    E = 3
+    u = torch.rand(E * 64, dtype=torch.float64, device=DEV)
+    G = torch.rand(E * 6 * 64, dtype=torch.float64, device=DEV)
+    with pytest.raises(ValueError):
+        ops.bk5(3, D, G, u, out=u)
+    B, Dm = (dev(a) for a in geo.bk13_bases(3))
+    G3 = torch.rand(E * 6 * 125, dtype=torch.float64, device=DEV)
+    with pytest.raises(ValueError):
+        ops.bk3(3, B, Dm, G3, u, out=u)
+    with pytest.raises(ValueError):
+        ops.bk1(3, B, torch.rand(E * 125, dtype=torch.float64, device=DEV), u, out=u)
 
 
 def _sample_check(kind, p, args_dev, out, nm, nq, sample):
