@@ -35,7 +35,7 @@ namespace {
 
 struct MixedTile {
     int32_t p;       // order
-    int32_t div;     // elements-per-workgroup divisor of that order's body (1 or 2)
+    int32_t div;     // elements-per-workgroup divisor of that order's body (1, 2 or 4)
     int32_t first;   // first index into the order-sorted element list
     int32_t count;   // elements in the tile
 };
@@ -151,6 +151,9 @@ __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
     __shared__ double smem[kMixedSmem];
     auto one = [&](int k, int tid) {
         const MixedTile t = tiles[k];
+        // a tile of divisor 4 runs on the DIV = 1 body: its slots past the
+        // tile's count stay idle, which is all a divisor changes here (the
+        // workgroup has 256 threads whatever the divisor)
         if (t.div == 2) {
             run_tile_p<2, LOOP, EO>(tid, t, uoff, goff, bases, boff, G, u, v, smem);
         } else {
@@ -448,6 +451,11 @@ int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
     if (nelmt > size_t(INT32_MAX)) {
         return BKX_ERROR_SIZE;
     }
+    for (int p = 1; p <= 8 && div_by_order; ++p) {
+        if (div_by_order[p] != 1 && div_by_order[p] != 2 && div_by_order[p] != 4) {
+            return BKX_ERROR_ARG;
+        }
+    }
     auto* pl = new (std::nothrow) bkx_mixed_plan;
     if (!pl) {
         return BKX_ERROR_DEVICE;
@@ -491,7 +499,7 @@ int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
     std::vector<double> cost;
     for (int p = 1; p <= 8; ++p) {
         const int q = bkx::q_of(p);
-        const int div = (div_by_order && div_by_order[p] == 2) ? 2 : 1;
+        const int div = div_by_order ? div_by_order[p] : 1;
         const int full = std::max(1, 256 / (q * q));
         const int epb = std::max(1, full / div);
         const int rounds = (rounds_by_order && rounds_by_order[p] > 0) ? rounds_by_order[p] : 1;

@@ -10,9 +10,10 @@ in [-1, 1]^{3x3}) and FP64 uniform[-1, 1] inputs.
 
 The batch runs as one launch over all orders or as one launch per order
 (each order by a kernel compiled for it alone), either captured in a
-hipGraph.  The elements per workgroup (divisor 1 or 2 of floor(256 / q^2))
-are tuned per order for each: ``tune_single`` for the single launch,
-``tune`` for the per-order launches.
+hipGraph.  The elements per workgroup (divisor 1, 2 or 4 of
+floor(256 / q^2)) are tuned per order for each: ``tune_single`` for the
+single launch, ``tune`` for the per-order launches (with the rows of G in
+flight).
 """
 import ctypes
 import statistics
@@ -364,8 +365,8 @@ def tune(order, bases, G, u, v, steps=10, plan=None):
 def tune_single(order, bases, G, u, v, steps=10):
     """For the single launch over all orders, whose kernel is sized for its
     largest order body: the divisors per order, chosen greedily over the
-    orders present, most elements first, each set to 2 when the whole launch
-    gets faster; returns (div_by_order, ms of the chosen plan)."""
+    orders present, most elements first, each set to 2 or 4 when the whole
+    launch gets faster; returns (div_by_order, ms of the chosen plan)."""
     counts = np.bincount(np.asarray(order), minlength=9)
 
     def timed(div):
@@ -378,9 +379,11 @@ def tune_single(order, bases, G, u, v, steps=10):
     best = [1] * 9
     best_ms = timed(best)
     for p in sorted((p for p in range(1, 9) if counts[p] > 0), key=lambda p: -counts[p]):
-        trial = list(best)
-        trial[p] = 2
-        ms = timed(trial)
-        if ms < best_ms:
-            best, best_ms = trial, ms
+        base = list(best)
+        for div in (2, 4):
+            trial = list(base)
+            trial[p] = div
+            ms = timed(trial)
+            if ms < best_ms:
+                best, best_ms = trial, ms
     return best, best_ms

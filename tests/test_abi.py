@@ -86,6 +86,11 @@ def test_error_paths_without_device():
         assert create(pbad, 2, None, None, 0, None) == 2
         assert create(None, 2, None, None, 0, ctypes.byref(h)) == 2
         assert create(pbad, 2, None, None, 0, ctypes.byref(h)) == 1
+        # elements-per-workgroup divisors other than 1, 2, 4: rejected before any device array
+        good = np.array([3, 5], dtype=np.int32)
+        pgood = good.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        div = np.array([0, 1, 1, 3, 1, 1, 1, 1, 1], dtype=np.int32)
+        assert create(pgood, 2, div.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), None, 0, ctypes.byref(h)) == 5
 
 
 def test_python_errors():

@@ -115,12 +115,24 @@ def test_tuned_single_launch_divisors():
     b, g, uu = _t(bases), _t(G), _t(u)
     v = torch.empty_like(uu)
     div, ms = mixed.tune_single(order, b, g, uu, v, steps=3)
-    assert len(div) == 9 and set(div) <= {1, 2} and ms > 0
+    assert len(div) == 9 and set(div) <= {1, 2, 4} and ms > 0
     tuned = mixed.MixedPlan(order, div_by_order=div)
     vt = torch.full_like(uu, float("nan"))
     tuned.apply(b, g, uu, vt)
     torch.cuda.synchronize()
     _oracle_check(order, tuned, bases, G, u, vt.cpu().numpy())
+    # every divisor, one workgroup per tile and persistent (three rounds a
+    # tile, so that workgroups loop): the same bits as divisor 1
+    ref = torch.full_like(uu, float("nan"))
+    plan.apply(b, g, uu, ref)
+    for d in (2, 4):
+        for rounds in (None, [3] * 9):
+            pl = mixed.MixedPlan(order, div_by_order=[d] * 9, rounds_by_order=rounds)
+            vd = torch.full_like(uu, float("nan"))
+            pl.apply(b, g, uu, vd)
+            torch.cuda.synchronize()
+            assert torch.equal(vd, ref), (d, rounds)
+            pl.close()
 
 
 def test_graph_replays():
