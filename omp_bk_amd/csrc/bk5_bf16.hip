@@ -19,7 +19,8 @@
 // operands, about 1e-5 relative.
 //
 // Mapping: one wavefront per element (n = 8: 512 points); u, the three
-// gradients (then fluxes) and the output accumulate in LDS, all in-wavefront.
+// gradients (then fluxes) and the output (in u's box) accumulate in LDS, all
+// in-wavefront.
 // The MFMA operand layout of v_mfma_f32_16x16x16_bf16 (64 lanes, 16 x 16 x 16):
 //   lane l holds A[l % 16][4 (l / 16) + 0..3], B[4 (l / 16) + 0..3][l % 16],
 //   and C[4 (l / 16) + 0..3][l % 16].
@@ -121,14 +122,19 @@ __device__ __forceinline__ void contract(const float* src, float* dst, int d, bf
     }
 }
 
-template <int WAVES>
+// G16: G is 16-byte aligned, and the fluxes read it as 16-byte vectors:
+// each lane takes the points (i, j, k0 .. k0 + 3) of two (i, j) rows and
+// loads G(e, i, j, f, k0 .. k0 + 3) for f = 0..5 (12 loads per lane instead
+// of 48 four-byte ones).
+template <int WAVES, bool G16>
 __global__ __launch_bounds__(kWave * WAVES) void bk5_bf16_kernel(size_t nelmt, const double* __restrict__ D,
                                                                 const float* __restrict__ G,
                                                                 const float* __restrict__ u, float* __restrict__ v)
 {
+    // u's box holds v once the gradients are formed (four boxes per
+    // wavefront: LDS then allows the four waves per SIMD the registers do)
     __shared__ float s_u[WAVES][BOX];
     __shared__ float s_q[WAVES][3][BOX];
-    __shared__ float s_v[WAVES][BOX];
 
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
@@ -136,18 +142,24 @@ __global__ __launch_bounds__(kWave * WAVES) void bk5_bf16_kernel(size_t nelmt, c
     const bool active = e < nelmt;
     const size_t ee = active ? e : 0;
     float* su = s_u[wave];
-    float* sv = s_v[wave];
+    float* sv = s_u[wave];
     const int j = lane / NQ;
     const int k = lane % NQ;
 
     const bf16x4 bg = b_operand(D, false, lane);   // gradient: D(i, n)
     const bf16x4 bt = b_operand(D, true, lane);    // transposed: D(n, i)
 
-    // u in the (j, k) plane of each layer i
+    // u in the (j, k) plane of each layer i (loaded unconditionally: an idle
+    // wavefront computes on element 0 and stores nothing)
     const float* ue = u + ee * NQ3 + lane;
+    float uv[NQ];
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
-        su[PX(i, j, k)] = active ? ue[i * NQ2] : 0.0f;
+        uv[i] = ld_stream(ue + i * NQ2);
+    }
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+        su[PX(i, j, k)] = uv[i];
     }
     wave_sync();
 #pragma unroll
@@ -155,20 +167,48 @@ __global__ __launch_bounds__(kWave * WAVES) void bk5_bf16_kernel(size_t nelmt, c
         contract<false>(su, s_q[wave][d], d, bg, lane);
     }
     wave_sync();
-    // fluxes, lane (j, k), layer i: G(e, i, j, f, k)
-    const float* ge = G + ee * 6 * NQ3 + j * 6 * NQ + k;
-#pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-        float g[6];
-#pragma unroll
-        for (int f = 0; f < 6; ++f) {
-            g[f] = active ? ge[i * 6 * NQ2 + f * NQ] : 0.0f;
-        }
-        const int pt = PX(i, j, k);
+    auto flux = [&](int pt, float g0, float g1, float g2, float g3, float g4, float g5) {
         const float qr = s_q[wave][0][pt], qs = s_q[wave][1][pt], qt = s_q[wave][2][pt];
-        s_q[wave][0][pt] = g[0] * qr + g[1] * qs + g[2] * qt;
-        s_q[wave][1][pt] = g[1] * qr + g[3] * qs + g[4] * qt;
-        s_q[wave][2][pt] = g[2] * qr + g[4] * qs + g[5] * qt;
+        s_q[wave][0][pt] = g0 * qr + g1 * qs + g2 * qt;
+        s_q[wave][1][pt] = g1 * qr + g3 * qs + g4 * qt;
+        s_q[wave][2][pt] = g2 * qr + g4 * qs + g5 * qt;
+    };
+    if constexpr (G16) {
+        // lane row q = lane + 64 r: (i, j) = (q / 16, (q / 2) % 8), k0 = 4 (q % 2)
+        const float* gE = G + ee * 6 * NQ3;
+        f32x4 g[2][6];
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int q = lane + kWave * r;
+            const int i = q / 16, jj = (q / 2) % NQ, k0 = 4 * (q % 2);
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                // ordinary loads: the six factor loads of a row share its
+                // lines (streamed, they ran at 123 against 175 GDoF/s)
+                g[r][f] = *reinterpret_cast<const f32x4*>(gE + ((i * NQ + jj) * 6 + f) * NQ + k0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+            const int q = lane + kWave * r;
+            const int i = q / 16, jj = (q / 2) % NQ, k0 = 4 * (q % 2);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                flux(PX(i, jj, k0 + t), g[r][0][t], g[r][1][t], g[r][2][t], g[r][3][t], g[r][4][t], g[r][5][t]);
+            }
+        }
+    } else {
+        // lane (j, k), layer i: G(e, i, j, f, k)
+        const float* ge = G + ee * 6 * NQ3 + j * 6 * NQ + k;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            float g[6];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) {
+                g[f] = ge[i * 6 * NQ2 + f * NQ];
+            }
+            flux(PX(i, j, k), g[0], g[1], g[2], g[3], g[4], g[5]);
+        }
     }
     wave_sync();
     contract<false>(s_q[wave][0], sv, 0, bt, lane);
@@ -181,7 +221,7 @@ __global__ __launch_bounds__(kWave * WAVES) void bk5_bf16_kernel(size_t nelmt, c
         float* ve = v + e * NQ3 + lane;
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
-            ve[i * NQ2] = sv[PX(i, j, k)];
+            st_stream(ve + i * NQ2, sv[PX(i, j, k)]);
         }
     }
 }
@@ -198,7 +238,13 @@ int bk5_apply_bf16s(int p, size_t nelmt, const double* D, const float* G, const 
     if (grid == 0) {
         return BKX_ERROR_SIZE;
     }
-    hipLaunchKernelGGL(bk5_bf16_kernel<waves>, dim3(grid), dim3(kWave * waves), 0, s, nelmt, D, G, u, v);
+    if (reinterpret_cast<uintptr_t>(G) % 16 == 0) {
+        hipLaunchKernelGGL((bk5_bf16_kernel<waves, true>), dim3(grid), dim3(kWave * waves), 0, s, nelmt, D, G, u,
+                           v);
+    } else {
+        hipLaunchKernelGGL((bk5_bf16_kernel<waves, false>), dim3(grid), dim3(kWave * waves), 0, s, nelmt, D, G, u,
+                           v);
+    }
     return launch_status();
 }
 
