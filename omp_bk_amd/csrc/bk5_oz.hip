@@ -36,18 +36,20 @@
 // (a = 0..5): ten MFMAs per direction and 32 cross points.  The A operands
 // (digits of D and D^T) are built once per wavefront.
 //
-// Flow per element: u loaded in each direction's layout (four values per
-// lane), split into digits, gradient MFMAs, recombined gradients (FP64) to
-// LDS; per direction of the divergence, the flux field it contracts
-// (rr = g0 qr + g1 qs + g2 qt for i, rs for j, rt for k) at that
-// direction's points, in FP64 from the gradients in LDS and G (FP32,
-// global), split into digits, divergence MFMAs, recombined and summed into
-// v in LDS; v stored in FP64.  FP64 is used for the recombination of the
+// Flow per element: u streamed once into LDS; per direction, u read in that
+// direction's layout (four values per lane), split into digits, gradient
+// MFMAs, recombined gradients (FP64) to LDS; one pointwise pass forms the
+// three flux fields (rr = g0 qr + g1 qs + g2 qt, rs, rt) in FP64 from the
+// gradients and G (FP32, global, read once) in place of the gradients; per
+// direction of the divergence, the field it contracts (rr for i, rs for j,
+// rt for k) read at that direction's points, split into digits, divergence
+// MFMAs, recombined and summed into v in LDS; v stored in FP64.  FP64 is used for the recombination of the
 // levels and for the pointwise fluxes (9 FMAs per point); every
 // contraction product is an exact BF16 product.
 //
 // Bytes per DoF: u 4 + G 24 (FP32) + v 8 (FP64) = 36, as the other FP32-
 // storage forms.
+#include <climits>
 #include <cmath>
 
 #include "bkx_device.hpp"
@@ -146,21 +148,38 @@ __device__ __forceinline__ float wave_max(float x)
     }
     return x;
 }
-__device__ __forceinline__ double wave_max(double x)
+
+// Points of direction d, N points per direction: line index n and cross
+// point x = (a, b) = (x / N, x % N) address point n sn + a sa + b sb, that is
+// (n, a, b) for d = 0, (a, n, b) for d = 1, (a, b, n) for d = 2.  The strides
+// are selected once per direction (uniform values, no branches per point).
+template <int N>
+struct OzDir {
+    int sn, sa, sb;
+    __device__ __forceinline__ explicit OzDir(int d)
+        : sn(d == 0 ? N * N : (d == 1 ? N : 1)), sa(d == 0 ? N : N * N), sb(d == 2 ? N : 1)
+    {
+    }
+    __device__ __forceinline__ int base(int x) const { return (x / N) * sa + (x % N) * sb; }
+};
+
+// max over the wavefront of frexp exponents (lanes whose values are all zero
+// pass INT_MIN), three at a time: the frexp exponent is nondecreasing in
+// |x|, so this is the exponent of the largest magnitude; 0 when every value
+// is zero, as exp_of(0)
+__device__ __forceinline__ void wave_max_exp3(int (&e)[3])
 {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
-        x = fmax(x, __shfl_xor(x, o));
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            e[c] = max(e[c], __shfl_xor(e[c], o));
+        }
     }
-    return x;
-}
-
-// point index (i, j, k) of line index n and cross point x = (a, b) in
-// direction d, N points per direction
-template <int N>
-__device__ __forceinline__ int oz_point(int d, int n, int x)
-{
-    return d == 0 ? n * N * N + x : (d == 1 ? (x / N) * N * N + n * N + x % N : x * N + n);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        e[c] = e[c] == INT_MIN ? 0 : e[c];
+    }
 }
 
 // x[k] for a lane-dependent k by selects (an index into a private array that
@@ -292,7 +311,9 @@ __device__ __forceinline__ void oz_operands(const double (&w)[4], int ew, bf16x4
     }
 }
 
-template <int N>
+// G16: G is 16-byte aligned (used at n = 8 only, where the flux pass reads G
+// as 16-byte vectors)
+template <int N, bool G16>
 __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const double* __restrict__ D,
                                                        const float* __restrict__ G, const float* __restrict__ u,
                                                        double* __restrict__ v)
@@ -313,31 +334,45 @@ __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const doubl
         const float* ue = u + e * N3;
         const float* ge = G + e * 6 * N3;
 
-        // one exponent for the element's u
+        // u staged once in v's box (free until the divergence), as floats,
+        // and one exponent for the element's u
+        float* su = reinterpret_cast<float*>(s_v);
+        constexpr int R = (N3 + kWave - 1) / kWave;
+        float ur[R];
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+            const int x = m * kWave + lane;
+            ur[m] = ld_stream(ue + ((kFull || x < N3) ? x : 0));
+        }
         float um = 0.0f;
 #pragma unroll
-        for (int m = 0; m < (N3 + kWave - 1) / kWave; ++m) {
+        for (int m = 0; m < R; ++m) {
             const int x = m * kWave + lane;
-            um = fmaxf(um, (kFull || x < N3) ? fabsf(ue[(kFull || x < N3) ? x : 0]) : 0.0f);
+            if (kFull || x < N3) {
+                su[x] = ur[m];
+                um = fmaxf(um, fabsf(ur[m]));
+            }
         }
         const int eu = exp_of(wave_max(um));
         const double uscale = ldexp(1.0, eu);
+        wave_sync();
 
         // gradient: qr, qs, qt (one direction and tile at a time: unrolled,
         // the loads and digits of all six would be live at once).  Line
         // indices n >= N and cross points x >= N^2 are padding: their values
-        // are zero (loaded from a valid address and masked) and their
-        // outputs are not stored.
+        // are zero and their outputs are not stored.
 #pragma unroll 1
         for (int d = 0; d < 3; ++d) {
+            const OzDir<N> dir(d);
 #pragma unroll 1
             for (int tile = 0; tile < 2; ++tile) {
                 const int x = 32 * tile + lane % 32;
+                const int b = dir.base(x) + 4 * h * dir.sn;
                 float w[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const bool ok = kFull || (4 * h + t < N && x < N2);
-                    const float val = ue[ok ? oz_point<N>(d, 4 * h + t, x) : 0];
+                    const float val = su[ok ? b + t * dir.sn : 0];
                     w[t] = ok ? val : 0.0f;
                 }
                 bf16x4 B[kDig];
@@ -347,53 +382,104 @@ __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const doubl
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     if (kFull || (4 * h + r < N && x < N2)) {
-                        s_q[d][oz_point<N>(d, 4 * h + r, x)] = q[r];
+                        s_q[d][b + r * dir.sn] = q[r];
                     }
                 }
             }
         }
         wave_sync();
 
-        // divergence: per direction, the flux it contracts at its own points
-#pragma unroll 1
-        for (int d = 0; d < 3; ++d) {
-            // factors of the flux of direction d: rr (0, 1, 2), rs (1, 3, 4), rt (2, 4, 5)
-            const int f0 = d == 2 ? 2 : d, f1 = d == 0 ? 1 : (d == 1 ? 3 : 4), f2 = d == 0 ? 2 : (d == 1 ? 4 : 5);
-            double f[2][4];
-            double fm = 0.0;
+        // the three flux fields (rr, rs, rt) = G (qr, qs, qt), pointwise, in
+        // place of the gradients, and each field's largest magnitude.  At
+        // n = 8 with G16 each lane takes the points (i, j, k0 .. k0 + 3) of
+        // two (i, j) rows and reads G(e, i, j, f, k0 .. k0 + 3) as 16-byte
+        // vectors (all twelve loads in flight at once); otherwise one point
+        // per lane and round, G read per point.
+        double fm[3] = {0.0, 0.0, 0.0};
+        auto flux = [&](int pt, float g0, float g1, float g2, float g3, float g4, float g5) {
+            const double q0 = s_q[0][pt], q1 = s_q[1][pt], q2 = s_q[2][pt];
+            const double rr = double(g0) * q0 + double(g1) * q1 + double(g2) * q2;
+            const double rs = double(g1) * q0 + double(g3) * q1 + double(g4) * q2;
+            const double rt = double(g2) * q0 + double(g4) * q1 + double(g5) * q2;
+            s_q[0][pt] = rr;
+            s_q[1][pt] = rs;
+            s_q[2][pt] = rt;
+            fm[0] = fmax(fm[0], fabs(rr));
+            fm[1] = fmax(fm[1], fabs(rs));
+            fm[2] = fmax(fm[2], fabs(rt));
+        };
+        if constexpr (kFull && G16) {
+            f32x4 g[2][6];
 #pragma unroll
-            for (int tile = 0; tile < 2; ++tile) {
-                const int x = 32 * tile + lane % 32;
+            for (int r = 0; r < 2; ++r) {
+                const int q = lane + kWave * r;
+                const int i = q / 16, jj = (q / 2) % N, k0 = 4 * (q % 2);
 #pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const bool ok = kFull || (4 * h + t < N && x < N2);
-                    const int pt = ok ? oz_point<N>(d, 4 * h + t, x) : 0;
-                    const int ii = pt / N2, jj = (pt / N) % N, kk = pt % N;
-                    const float* gp = ge + (ii * N + jj) * 6 * N + kk;
-                    const double val = double(gp[f0 * N]) * s_q[0][pt] + double(gp[f1 * N]) * s_q[1][pt] +
-                                       double(gp[f2 * N]) * s_q[2][pt];
-                    f[tile][t] = ok ? val : 0.0;
-                    fm = fmax(fm, fabs(f[tile][t]));
+                for (int f = 0; f < 6; ++f) {
+                    g[r][f] = *reinterpret_cast<const f32x4*>(ge + ((i * N + jj) * 6 + f) * N + k0);
                 }
             }
-            const int ef = exp_of(wave_max(fm));
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+                const int q = lane + kWave * r;
+                const int i = q / 16, jj = (q / 2) % N, k0 = 4 * (q % 2);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    flux((i * N + jj) * N + k0 + t, g[r][0][t], g[r][1][t], g[r][2][t], g[r][3][t], g[r][4][t],
+                         g[r][5][t]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int m = 0; m < R; ++m) {
+                const int x = m * kWave + lane;
+                const int pt = (kFull || x < N3) ? x : 0;
+                const int ii = pt / N2, jj = (pt / N) % N, kk = pt % N;
+                const float* gp = ge + (ii * N + jj) * 6 * N + kk;
+                float g[6];
+#pragma unroll
+                for (int f = 0; f < 6; ++f) {
+                    g[f] = gp[f * N];
+                }
+                if (kFull || x < N3) {
+                    flux(pt, g[0], g[1], g[2], g[3], g[4], g[5]);
+                }
+            }
+        }
+        wave_sync();
+
+        // divergence: per direction, the flux field it contracts, read at its
+        // own points
+        int efs[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            efs[c] = fm[c] > 0.0 ? exp_of(fm[c]) : INT_MIN;
+        }
+        wave_max_exp3(efs);
+#pragma unroll 1
+        for (int d = 0; d < 3; ++d) {
+            const OzDir<N> dir(d);
+            const int ef = d == 0 ? efs[0] : (d == 1 ? efs[1] : efs[2]);
             const double fscale = ldexp(1.0, ef);
 #pragma unroll 1
             for (int tile = 0; tile < 2; ++tile) {
                 const int x = 32 * tile + lane % 32;
-                bf16x4 B[kDig];
+                const int b = dir.base(x) + 4 * h * dir.sn;
                 double ft[4];
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
-                    ft[t] = tile == 0 ? f[0][t] : f[1][t];
+                    const bool ok = kFull || (4 * h + t < N && x < N2);
+                    const double val = s_q[d][ok ? b + t * dir.sn : 0];
+                    ft[t] = ok ? val : 0.0;
                 }
+                bf16x4 B[kDig];
                 oz_operands(ft, ef, B);
                 double o[4];
                 oz_contract(Md, B, fscale, o);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     if (kFull || (4 * h + r < N && x < N2)) {
-                        const int pt = oz_point<N>(d, 4 * h + r, x);
+                        const int pt = b + r * dir.sn;
                         s_v[pt] = d == 0 ? o[r] : s_v[pt] + o[r];
                     }
                 }
@@ -424,11 +510,11 @@ int bk5_apply_oz(int p, size_t nelmt, const double* D, const float* G, const flo
     };
     switch (p) {
     case 5:
-        return go(bk5_oz_kernel<6>);
+        return go(bk5_oz_kernel<6, false>);
     case 6:
-        return go(bk5_oz_kernel<7>);
+        return go(bk5_oz_kernel<7, false>);
     case 7:
-        return go(bk5_oz_kernel<8>);
+        return reinterpret_cast<uintptr_t>(G) % 16 == 0 ? go(bk5_oz_kernel<8, true>) : go(bk5_oz_kernel<8, false>);
     default:
         return BKX_ERROR_ORDER;
     }
