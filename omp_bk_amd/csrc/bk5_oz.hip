@@ -312,17 +312,33 @@ __device__ __forceinline__ void oz_operands(const double (&w)[4], int ew, bf16x4
 }
 
 // G16: G is 16-byte aligned (used at n = 8 only, where the flux pass reads G
-// as 16-byte vectors)
+// as 16-byte vectors).  Registers are held to three wavefronts per SIMD
+// (at n = 8, 168 VGPRs: the only spills are in the setup of the digit
+// matrices, before the element loop), and 24 N^3 bytes of LDS per
+// wavefront allow at least 3.25.
 template <int N, bool G16>
-__global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const double* __restrict__ D,
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void bk5_oz_kernel(size_t nelmt, const double* __restrict__ D,
                                                        const float* __restrict__ G, const float* __restrict__ u,
                                                        double* __restrict__ v)
 {
     constexpr int N2 = N * N;
     constexpr int N3 = N2 * N;
     constexpr bool kFull = N == kK;   // no padding: every mask below is true
-    __shared__ double s_q[3][N3];   // qr, qs, qt of the element
-    __shared__ double s_v[N3];      // v of the element
+    // qr, qs, qt of the element; then, in their place, rr, rs, rt.  Two
+    // more fields share these boxes:
+    // - v, in the box of qr / rr.  Each divergence tile reads all of its
+    //   lines of the field it contracts (the 8 line points of each of its
+    //   cross points) before it writes v at the same points, and the tiles
+    //   cover disjoint lines: direction i overwrites rr only where it has
+    //   already read it.
+    // - u, as floats, in the top half of the box of qt (floats N3 .. 2 N3 - 1
+    //   of it).  Only qt's direction writes that box.  Its first tile
+    //   (cross points x < 32, points x N + n < 32 N) writes bytes below
+    //   256 N; its second tile reads u at points >= 32 N, bytes
+    //   4 N3 + 128 N and up (>= 256 N for N >= 6), before it writes.
+    __shared__ double s_q[3][N3];
+    double* const s_v = s_q[0];
+    float* const su = reinterpret_cast<float*>(s_q[2]) + N3;
 
     const int lane = threadIdx.x;
     const int h = lane / 32;
@@ -334,9 +350,8 @@ __global__ __launch_bounds__(kWave) void bk5_oz_kernel(size_t nelmt, const doubl
         const float* ue = u + e * N3;
         const float* ge = G + e * 6 * N3;
 
-        // u staged once in v's box (free until the divergence), as floats,
-        // and one exponent for the element's u
-        float* su = reinterpret_cast<float*>(s_v);
+        // u staged once in LDS, as floats, and one exponent for the
+        // element's u
         constexpr int R = (N3 + kWave - 1) / kWave;
         float ur[R];
 #pragma unroll
