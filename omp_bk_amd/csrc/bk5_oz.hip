@@ -22,19 +22,23 @@
 // is below 2^-44 of the operands' scale.  Measured against the FP64
 // operator on the same FP32 inputs: ~2e-14 max-relative.
 //
-// MFMA mapping (v_mfma_f32_32x32x8bf16_1k, one wavefront per element): rows
+// MFMA mapping (v_mfma_f32_32x32x16_bf16, one wavefront per element): rows
 // of A and C are (level group g, output i), 4 x 8; columns of B and C are 32
-// cross points x (two tiles cover the n^2 <= 64 cross points); K = 8 is the
-// contraction index n.  For n = 6, 7 (p = 5, 6) the rows, columns and K
-// beyond n carry zero digits and their outputs are not stored.  Lane l holds
-// A[l % 32][4 (l / 32) + 0..3], B[4 (l / 32) + 0..3][l % 32] and
-// C[8 (r / 4) + 4 (l / 32) + r % 4][l % 32], r = 0..15, so that every lane
-// holds all level groups of its four outputs: the recombination is local to
-// the lane.  Per digit a of w, one MFMA adds A_a((g, i), n) = dM_{g - a}(i, n)
-// into the accumulator of levels 0..3 (a = 0..3), and one adds
-// A'_a((g, i), n) = dM_{4 + g - a}(i, n) (g = 0, 1) into that of levels 4, 5
-// (a = 0..5): ten MFMAs per direction and 32 cross points.  The A operands
-// (digits of D and D^T) are built once per wavefront.
+// cross points x (two tiles cover the n^2 <= 64 cross points); K = 16 is
+// (digit parity s, contraction index n).  For n = 6, 7 (p = 5, 6) the rows,
+// columns and K beyond n carry zero digits and their outputs are not
+// stored.  Lane l (h = l / 32) holds C[8 (r / 4) + 4 h + r % 4][l % 32],
+// r = 0..15, so that every lane holds all level groups of its four outputs:
+// the recombination is local to the lane.  Its eight K slots of B carry
+// digits a and a + 1 of its four values w(4 h + 0..3, l % 32), and its eight
+// K slots of A the digits of M(row l % 32, 4 h + 0..3) that pair with them,
+// so each MFMA adds the products of two digits a, a + 1 of w (the two halves
+// of K):
+//   A_a((g, i), n) = dM_{g - a}(i, n)           into levels 0..3 (a = 0..3),
+//   A'_a((g, i), n) = dM_{4 + g - a}(i, n), g < 2, into levels 4, 5 (a = 0..5):
+// two plus three MFMAs per direction and 32 cross points.  Which K slots the
+// hardware gives a lane does not matter: A and B use the same ones.  The A
+// operands (digits of D and D^T) are built once per wavefront.
 //
 // Flow per element: u streamed once into LDS; per direction, u read in that
 // direction's layout (four values per lane), split into digits, gradient
@@ -59,6 +63,8 @@ namespace bkx {
 namespace {
 
 using bf16x4 = short __attribute__((ext_vector_type(4)));
+using s16x8 = short __attribute__((ext_vector_type(8)));
+using bf16x8 = __bf16 __attribute__((ext_vector_type(8)));
 using fx16 = float __attribute__((ext_vector_type(16)));
 using bf16x2 = __bf16 __attribute__((ext_vector_type(2)));
 using fx2 = float __attribute__((ext_vector_type(2)));
@@ -140,6 +146,12 @@ __device__ __forceinline__ bf16x4 bf16_quad(float a, float b, float c, float d)
     return __builtin_bit_cast(bf16x4, (uint64_t(hi) << 32) | lo);
 }
 
+// the eight K slots of a lane: the four values of x, then those of y
+__device__ __forceinline__ bf16x8 k_pair(bf16x4 x, bf16x4 y)
+{
+    return __builtin_bit_cast(bf16x8, s16x8(__builtin_shufflevector(x, y, 0, 1, 2, 3, 4, 5, 6, 7)));
+}
+
 __device__ __forceinline__ float wave_max(float x)
 {
 #pragma unroll
@@ -200,8 +212,8 @@ __device__ __forceinline__ T pick(const T (&x)[K], int k)
 // its four output rows: lane l holds row (g, i) = l % 32 of A, columns
 // n = 4 h + 0..3 (h = l / 32), and outputs i = 4 h + 0..3.
 struct OzMat {
-    bf16x4 lo[kLoLevels];   // A_a, a = 0..3 (levels 0..3)
-    bf16x4 hi[kDig];        // A'_a, a = 0..5 (levels 4, 5)
+    bf16x8 lo[kLoLevels / 2];   // (A_a, A_{a+1}), a = 0, 2 (levels 0..3)
+    bf16x8 hi[kDig / 2];        // (A'_a, A'_{a+1}), a = 0, 2, 4 (levels 4, 5)
     double scale[4];        // 2^{e_M(i) - 16} of the lane's output rows
 
     template <bool TR, int N>
@@ -226,19 +238,28 @@ struct OzMat {
         for (int t = 0; t < 4; ++t) {
             digits(ldexp(M(i, 4 * h + t), -ei), dig[t]);
         }
+        bf16x4 l4[kLoLevels], h4[kDig];
 #pragma unroll
         for (int a = 0; a < kLoLevels; ++a) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                lo[a][t] = bf16_of(float(pick(dig[t], g - a)));
+                l4[a][t] = bf16_of(float(pick(dig[t], g - a)));
             }
         }
 #pragma unroll
         for (int a = 0; a < kDig; ++a) {
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                hi[a][t] = g < 2 ? bf16_of(float(pick(dig[t], 4 + g - a))) : short(0);
+                h4[a][t] = g < 2 ? bf16_of(float(pick(dig[t], 4 + g - a))) : short(0);
             }
+        }
+#pragma unroll
+        for (int a = 0; a < kLoLevels / 2; ++a) {
+            lo[a] = k_pair(l4[2 * a], l4[2 * a + 1]);
+        }
+#pragma unroll
+        for (int a = 0; a < kDig / 2; ++a) {
+            hi[a] = k_pair(h4[2 * a], h4[2 * a + 1]);
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -270,12 +291,12 @@ __device__ __forceinline__ void oz_contract(const OzMat& M, const bf16x4 (&B)[kD
     fx16 c_lo = {};
     fx16 c_hi = {};
 #pragma unroll
-    for (int a = 0; a < kLoLevels; ++a) {
-        c_lo = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(M.lo[a], B[a], c_lo, 0, 0, 0);
+    for (int a = 0; a < kLoLevels / 2; ++a) {
+        c_lo = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M.lo[a], k_pair(B[2 * a], B[2 * a + 1]), c_lo, 0, 0, 0);
     }
 #pragma unroll
-    for (int a = 0; a < kDig; ++a) {
-        c_hi = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(M.hi[a], B[a], c_hi, 0, 0, 0);
+    for (int a = 0; a < kDig / 2; ++a) {
+        c_hi = __builtin_amdgcn_mfma_f32_32x32x16_bf16(M.hi[a], k_pair(B[2 * a], B[2 * a + 1]), c_hi, 0, 0, 0);
     }
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
