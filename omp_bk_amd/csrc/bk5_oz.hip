@@ -393,32 +393,40 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
         const double uscale = ldexp(1.0, eu);
         wave_sync();
 
-        // gradient: qr, qs, qt (one direction and tile at a time: unrolled,
-        // the loads and digits of all six would be live at once).  Line
-        // indices n >= N and cross points x >= N^2 are padding: their values
-        // are zero and their outputs are not stored.
+        // gradient: qr, qs, qt, one direction at a time (unrolled, the loads
+        // and digits of all six tiles would be live at once).  Both tiles of
+        // a direction read their values first, so that the second tile's
+        // digits can be formed while the first one's products run (and no
+        // read comes after a write of the direction: qt's writes reach the
+        // u staging).  Line indices n >= N and cross points x >= N^2 are
+        // padding: their values are zero and their outputs are not stored.
 #pragma unroll 1
         for (int d = 0; d < 3; ++d) {
             const OzDir<N> dir(d);
-#pragma unroll 1
+            float w[2][4];
+            int b[2];
+#pragma unroll
             for (int tile = 0; tile < 2; ++tile) {
                 const int x = 32 * tile + lane % 32;
-                const int b = dir.base(x) + 4 * h * dir.sn;
-                float w[4];
+                b[tile] = dir.base(x) + 4 * h * dir.sn;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const bool ok = kFull || (4 * h + t < N && x < N2);
-                    const float val = su[ok ? b + t * dir.sn : 0];
-                    w[t] = ok ? val : 0.0f;
+                    const float val = su[ok ? b[tile] + t * dir.sn : 0];
+                    w[tile][t] = ok ? val : 0.0f;
                 }
+            }
+#pragma unroll
+            for (int tile = 0; tile < 2; ++tile) {
+                const int x = 32 * tile + lane % 32;
                 bf16x4 B[kDig];
-                oz_operands(w, eu, B);
+                oz_operands(w[tile], eu, B);
                 double q[4];
                 oz_contract(Mg, B, uscale, q);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     if (kFull || (4 * h + r < N && x < N2)) {
-                        s_q[d][b + r * dir.sn] = q[r];
+                        s_q[d][b[tile] + r * dir.sn] = q[r];
                     }
                 }
             }
@@ -497,25 +505,30 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
             const OzDir<N> dir(d);
             const int ef = d == 0 ? efs[0] : (d == 1 ? efs[1] : efs[2]);
             const double fscale = ldexp(1.0, ef);
-#pragma unroll 1
+            double ft[2][4];
+            int b[2];
+#pragma unroll
             for (int tile = 0; tile < 2; ++tile) {
                 const int x = 32 * tile + lane % 32;
-                const int b = dir.base(x) + 4 * h * dir.sn;
-                double ft[4];
+                b[tile] = dir.base(x) + 4 * h * dir.sn;
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                     const bool ok = kFull || (4 * h + t < N && x < N2);
-                    const double val = s_q[d][ok ? b + t * dir.sn : 0];
-                    ft[t] = ok ? val : 0.0;
+                    const double val = s_q[d][ok ? b[tile] + t * dir.sn : 0];
+                    ft[tile][t] = ok ? val : 0.0;
                 }
+            }
+#pragma unroll
+            for (int tile = 0; tile < 2; ++tile) {
+                const int x = 32 * tile + lane % 32;
                 bf16x4 B[kDig];
-                oz_operands(ft, ef, B);
+                oz_operands(ft[tile], ef, B);
                 double o[4];
                 oz_contract(Md, B, fscale, o);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     if (kFull || (4 * h + r < N && x < N2)) {
-                        const int pt = b + r * dir.sn;
+                        const int pt = b[tile] + r * dir.sn;
                         s_v[pt] = d == 0 ? o[r] : s_v[pt] + o[r];
                     }
                 }
