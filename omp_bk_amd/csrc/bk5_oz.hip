@@ -47,9 +47,9 @@
 // gradients and G (FP32, global, read once) in place of the gradients; per
 // direction of the divergence, the field it contracts (rr for i, rs for j,
 // rt for k) read at that direction's points, split into digits, divergence
-// MFMAs, recombined and summed into v in LDS; v stored in FP64.  FP64 is used for the recombination of the
-// levels and for the pointwise fluxes (9 FMAs per point); every
-// contraction product is an exact BF16 product.
+// MFMAs, recombined and summed into v in LDS; v stored in FP64.  FP64 is
+// used for the recombination of the levels and for the pointwise fluxes (9
+// FMAs per point); every contraction product is an exact BF16 product.
 //
 // Bytes per DoF: u 4 + G 24 (FP32) + v 8 (FP64) = 36, as the other FP32-
 // storage forms.
@@ -72,7 +72,7 @@ using fx2 = float __attribute__((ext_vector_type(2)));
 constexpr int kK = 8;          // MFMA K and output rows: points per direction, padded
 constexpr int kDig = 6;        // digits per operand
 constexpr int kLoLevels = 4;   // levels 0..3 in the first accumulator
-constexpr int kOzGrid = 4096;  // workgroups (one wavefront each), looping over elements
+constexpr int kOzGrid = 3072;  // workgroups when the runtime cannot report the resident count
 
 // frexp exponent: |x| / 2^e in [0.5, 1) (0 for x = 0)
 __device__ __forceinline__ int exp_of(float m)
@@ -550,20 +550,35 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
 
 }  // namespace
 
+namespace {
+
+// One wavefront per workgroup, looping over the elements; as many workgroups
+// as are resident at once (three per SIMD at every order), so that none
+// waits for another to finish its elements.
+template <auto Kernel>
+int launch_oz(size_t nelmt, const double* D, const float* G, const float* u, double* v, hipStream_t s)
+{
+    unsigned slots = resident_blocks<Kernel>(kWave);
+    if (slots == 0) {
+        slots = kOzGrid;
+    }
+    const unsigned grid = unsigned(nelmt < size_t(slots) ? nelmt : size_t(slots));
+    hipLaunchKernelGGL(Kernel, dim3(grid), dim3(kWave), 0, s, nelmt, D, G, u, v);
+    return launch_status();
+}
+
+}  // namespace
+
 int bk5_apply_oz(int p, size_t nelmt, const double* D, const float* G, const float* u, double* v, hipStream_t s)
 {
-    const unsigned grid = unsigned(nelmt < size_t(kOzGrid) ? nelmt : size_t(kOzGrid));
-    auto go = [&](auto kernel) {
-        hipLaunchKernelGGL(kernel, dim3(grid), dim3(kWave), 0, s, nelmt, D, G, u, v);
-        return launch_status();
-    };
     switch (p) {
     case 5:
-        return go(bk5_oz_kernel<6, false>);
+        return launch_oz<bk5_oz_kernel<6, false>>(nelmt, D, G, u, v, s);
     case 6:
-        return go(bk5_oz_kernel<7, false>);
+        return launch_oz<bk5_oz_kernel<7, false>>(nelmt, D, G, u, v, s);
     case 7:
-        return reinterpret_cast<uintptr_t>(G) % 16 == 0 ? go(bk5_oz_kernel<8, true>) : go(bk5_oz_kernel<8, false>);
+        return reinterpret_cast<uintptr_t>(G) % 16 == 0 ? launch_oz<bk5_oz_kernel<8, true>>(nelmt, D, G, u, v, s)
+                                                        : launch_oz<bk5_oz_kernel<8, false>>(nelmt, D, G, u, v, s);
     default:
         return BKX_ERROR_ORDER;
     }
