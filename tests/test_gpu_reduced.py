@@ -198,3 +198,27 @@ def test_oz_rejects_other_orders():
     with pytest.raises(Exception):
         ops.bk5(4, torch.zeros(25, dtype=torch.float64, device=DEV), torch.zeros(6 * 125, device=DEV),
                 torch.zeros(125, device=DEV), form="oz")
+
+
+@pytest.mark.parametrize("form", ["oz", "bf16s"])
+@pytest.mark.parametrize("E", [37, 4097])
+def test_unaligned_G_same_bits(form, E):
+    """At p = 7 the oz and bf16s kernels read G as 16-byte vectors when G is
+    16-byte aligned, and point by point otherwise: G at an address that is
+    4-byte but not 16-byte aligned gives the same bits as the aligned copy.
+    E = 4097 makes workgroups loop over elements."""
+    p = 7
+    D, G, u = _case(p, E, seed=900 + E)
+    Dt = torch.as_tensor(D, device=DEV)
+    Ga = torch.as_tensor(G.astype(np.float32), device=DEV)
+    Gs = torch.empty(Ga.numel() + 1, dtype=torch.float32, device=DEV)[1:]
+    Gs.copy_(Ga)
+    assert Ga.data_ptr() % 16 == 0 and Gs.data_ptr() % 16 != 0
+    u32 = torch.as_tensor(u.astype(np.float32), device=DEV)
+    odt = torch.float32 if form == "bf16s" else torch.float64
+    a = torch.full((u.size,), float("nan"), dtype=odt, device=DEV)
+    b = torch.full_like(a, float("nan"))
+    ops.bk5(p, Dt, Ga, u32, out=a, form=form)
+    ops.bk5(p, Dt, Gs, u32, out=b, form=form)
+    assert bool(torch.isfinite(a).all().item())
+    assert torch.equal(a, b)
