@@ -334,13 +334,12 @@ __device__ __forceinline__ void oz_operands(const double (&w)[4], int ew, bf16x4
 
 // G16: G is 16-byte aligned (used at n = 8 only, where the flux pass reads G
 // as 16-byte vectors).  Registers are held to three wavefronts per SIMD
-// (at n = 8, 168 VGPRs: the only spills are in the setup of the digit
-// matrices, before the element loop), and 24 N^3 bytes of LDS per
-// wavefront allow at least 3.25.
+// (159 VGPRs at n = 8, no spills), and 24 N^3 bytes of LDS per wavefront
+// allow at least 3.25.
 template <int N, bool G16>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void bk5_oz_kernel(size_t nelmt, const double* __restrict__ D,
-                                                       const float* __restrict__ G, const float* __restrict__ u,
-                                                       double* __restrict__ v)
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void bk5_oz_kernel(
+    size_t nelmt, const double* __restrict__ D, const float* __restrict__ G, const float* __restrict__ u,
+    double* __restrict__ v)
 {
     constexpr int N2 = N * N;
     constexpr int N3 = N2 * N;
@@ -353,10 +352,8 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(3))) void
     //   cover disjoint lines: direction i overwrites rr only where it has
     //   already read it.
     // - u, as floats, in the top half of the box of qt (floats N3 .. 2 N3 - 1
-    //   of it).  Only qt's direction writes that box.  Its first tile
-    //   (cross points x < 32, points x N + n < 32 N) writes bytes below
-    //   256 N; its second tile reads u at points >= 32 N, bytes
-    //   4 N3 + 128 N and up (>= 256 N for N >= 6), before it writes.
+    //   of it).  Only qt's direction writes that box, and it reads all of
+    //   its u values (both tiles) before it writes any.
     __shared__ double s_q[3][N3];
     double* const s_v = s_q[0];
     float* const su = reinterpret_cast<float*>(s_q[2]) + N3;
