@@ -134,11 +134,10 @@ int bkx_bk5_apply_bf16s(int p, size_t nelmt, const double* dbasis, const float* 
  * 8 deep, so n < 8 is zero-padded; BKX_ERROR_ORDER otherwise): every contraction is
  * evaluated from six 8-bit integer digits of each operand (D per row, u and
  * each flux field per element, each digit exact in BF16) whose products are
- * summed per digit level by v_mfma_f32_32x32x16_bf16 (two digits of u or the
- * flux per product), exactly (every level sum is an integer below 2^24); the
- * levels are recombined once per output in
- * FP64, as are the pointwise fluxes.  u and G FP32, v FP64 (36 bytes per
- * DoF).  Error vs the FP64 operator on the same FP32 inputs ~1e-14 relative.
+ * summed per digit level by v_mfma_f32_32x32x16_bf16 (two digits of u or of
+ * a flux field per instruction), exactly (every level sum is an integer below
+ * 2^24); the levels are recombined once per output in FP64, as are the
+ * pointwise fluxes.  u and G FP32, v FP64 (36 bytes per DoF).  Error vs the FP64 operator on the same FP32 inputs ~1e-14 relative.
  * No reference counterpart. */
 int bkx_bk5_apply_oz(int p, size_t nelmt, const double* dbasis, const float* G, const float* in, double* out,
                      bkx_stream_t stream);
