@@ -371,8 +371,10 @@ typedef struct bkx_graph bkx_graph;
 /* Builds a plan (host work plus a few small device arrays): the elements
  * sorted by order and cut into tiles of rounds_by_order[p] x epb elements,
  * epb = floor(256 / q^2) / div_by_order[p] (div 1, 2 or 4, at least one
- * element; BKX_ERROR_ARG for another value), kept both grouped
- * by order and sorted by decreasing cost.  div_by_order and rounds_by_order
+ * element; BKX_ERROR_ARG for another value), kept both grouped by order
+ * and in launch order: by decreasing cost for the persistent launch, by the
+ * memory position of the tile's first element (last first) for one
+ * workgroup per tile.  div_by_order and rounds_by_order
  * are indexed by the order: each holds 9 entries, [0] unused and [p] for
  * p = 1..8 (a shorter array is read out of bounds); either may be NULL for
  * all 1.  grid <= 0 picks the number of resident workgroups of the device.

@@ -13,11 +13,12 @@
 // offsets at its position in the sorted list.  The single launch cuts each
 // order's part of the list into tiles of rounds x epb elements (epb =
 // elements per workgroup of that order's kernel body, chosen per order by
-// the tuner), sorted by decreasing cost (persistent: workgroup g takes tiles
-// g, g + grid, ...: a static longest-first assignment, no atomics, no
-// counters to reset).  The per-order launches run each order's part of the
-// list with a kernel compiled for that order alone.  Both run the
-// transposing BK3 body of bk13t_body.hpp.
+// the tuner).  The persistent loop takes them by decreasing cost (workgroup
+// g takes tiles g, g + grid, ...: a static longest-first assignment, no
+// atomics, no counters to reset); one workgroup per tile takes them by the
+// memory position of their elements.  The per-order launches run each
+// order's part of the list with a kernel compiled for that order alone.
+// Both run the transposing BK3 body of bk13t_body.hpp.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -136,7 +137,7 @@ __device__ __forceinline__ void run_tile_p(int tid, const MixedTile& t, const in
 }
 
 // LOOP = false: one workgroup per tile (a single launch over all orders; the
-// hardware dispatches the workgroups, longest tiles first, as CUs free up).
+// hardware dispatches the workgroups in tile order as CUs free up).
 // LOOP = true: persistent, workgroup g processes tiles g, g + gridDim.x, ...
 // In the persistent form the basis rows are read with vector loads (loads
 // after the loop's barriers and stores cannot be proven unclobbered, so they
@@ -230,7 +231,7 @@ struct bkx_mixed_plan {
     int64_t boff_h[10] = {};            // per order: offset of B_p in the bases array
     int64_t eoff_h[10] = {};            // per order: offset of B_p's even-odd form in the prepared array
     std::vector<int64_t> uoff_h, goff_h;
-    bkx::MixedTile* tiles_persist = nullptr;
+    bkx::MixedTile* tiles = nullptr;
     int64_t* uoff = nullptr;    // by position in the order-sorted element list
     int64_t* goff = nullptr;
     int64_t* boff = nullptr;
@@ -341,7 +342,7 @@ int apply_single(const bkx_mixed_plan* plan, const double* bases, const double* 
     if (!bases || !G || !in || !out) {
         return BKX_ERROR_NULL;
     }
-    return launch_tiles<EO>(plan->tiles_persist, plan->ntiles, plan->persistent, plan, bases, G, in, out, s);
+    return launch_tiles<EO>(plan->tiles, plan->ntiles, plan->persistent, plan, bases, G, in, out, s);
 }
 
 template <bool EO>
@@ -530,13 +531,6 @@ int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
         goff_s[i] = pl->goff_h[size_t(elems[i])];
     }
     pl->ntiles = int(grouped.size());
-    std::vector<int> idx(grouped.size());
-    std::iota(idx.begin(), idx.end(), 0);
-    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<bkx::MixedTile> persist(grouped.size());
-    for (size_t k = 0; k < idx.size(); ++k) {
-        persist[k] = grouped[idx[k]];
-    }
     int64_t boff[10] = {};
     int64_t bo = 0;
     for (int p = 1; p <= 8; ++p) {
@@ -570,16 +564,36 @@ int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
     }
     // persistent when asked for: rounds of more than one workgroup's
     // elements per tile, or a grid given without rounds.  Otherwise one
-    // workgroup per tile, the hardware dispatching them longest first as
-    // CUs free up (on MI355X 54 GDoF/s for the 200,000-element Zipf batch,
-    // against 49 for the persistent loop, whose basis rows are vector loads).
+    // workgroup per tile, the hardware dispatching them in list order as CUs
+    // free up (on MI355X 59.7 GDoF/s for the 200,000-element Zipf batch in
+    // the caller's layout, against 50.5 for the persistent loop).
     if (grid_given && rounds_by_order == nullptr) {
         pl->persistent = true;
     }
     pl->grid = std::max(1, std::min(grid, std::max(1, pl->ntiles)));
 
+    // Tile order.  Persistent: longest first, so that the static split
+    // (workgroup g takes tiles g, g + grid, ...) balances.  One workgroup
+    // per tile: by the memory position of the tile's first element, last
+    // first.  The workgroups in flight then work on neighbouring elements:
+    // in the caller's layout, which interleaves the orders, that ran at 59.7
+    // against 56.5 GDoF/s longest first.  In the grouped layout the orders
+    // lie in ascending order in memory, so this is longest first as well.
+    std::vector<int> idx(grouped.size());
+    std::iota(idx.begin(), idx.end(), 0);
+    if (pl->persistent) {
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    } else {
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int a, int b) { return uoff_s[grouped[a].first] > uoff_s[grouped[b].first]; });
+    }
+    std::vector<bkx::MixedTile> ordered(grouped.size());
+    for (size_t k = 0; k < idx.size(); ++k) {
+        ordered[k] = grouped[idx[k]];
+    }
+
     int st = BKX_SUCCESS;
-    st = st ? st : upload(&pl->tiles_persist, persist.data(), persist.size());
+    st = st ? st : upload(&pl->tiles, ordered.data(), ordered.size());
     st = st ? st : upload(&pl->uoff, uoff_s.data(), nelmt);
     st = st ? st : upload(&pl->goff, goff_s.data(), nelmt);
     st = st ? st : upload(&pl->boff, boff, 10);
@@ -601,7 +615,7 @@ int bkx_mixed_plan_destroy(bkx_mixed_plan* plan)
     if (!plan) {
         return BKX_SUCCESS;
     }
-    for (void* ptr : {static_cast<void*>(plan->tiles_persist),
+    for (void* ptr : {static_cast<void*>(plan->tiles),
                       static_cast<void*>(plan->uoff),
                       static_cast<void*>(plan->goff), static_cast<void*>(plan->boff),
                       static_cast<void*>(plan->eoff)}) {
