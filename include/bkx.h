@@ -372,15 +372,16 @@ typedef struct bkx_graph bkx_graph;
  * sorted by order and cut into tiles of rounds_by_order[p] x epb elements,
  * epb = floor(256 / q^2) / div_by_order[p] (div 1, 2 or 4, at least one
  * element; BKX_ERROR_ARG for another value), kept both grouped by order
- * and in launch order: by decreasing cost for the persistent launch, by the
- * memory position of the tile's first element (last first) for one
- * workgroup per tile.  div_by_order and rounds_by_order
- * are indexed by the order: each holds 9 entries, [0] unused and [p] for
- * p = 1..8 (a shorter array is read out of bounds); either may be NULL for
- * all 1.  grid <= 0 picks the number of resident workgroups of the device.
- * The single launch is persistent (workgroup g takes tiles g, g + grid, ...)
- * when some rounds_by_order[p] > 1, or when grid > 0 and rounds_by_order is
- * NULL; otherwise it runs one workgroup per tile. */
+ * and in launch order (by the memory position of the tile's first element,
+ * last first).  div_by_order and rounds_by_order are indexed by the order:
+ * each holds 9 entries, [0] unused and [p] for p = 1..8 (a shorter array is
+ * read out of bounds); either may be NULL for all 1.  grid <= 0 picks the
+ * number of resident workgroups of the device.  The single launch is
+ * persistent (grid workgroups, each taking the next tile from a queue held
+ * by the plan) when some rounds_by_order[p] > 1, or when grid > 0 and
+ * rounds_by_order is NULL; otherwise it runs one workgroup per tile.  The
+ * queue is two counters that the launch leaves at zero: persistent launches
+ * with one plan (direct or replayed from a graph) must not overlap. */
 int bkx_mixed_plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
                           const int32_t* rounds_by_order, int grid, bkx_mixed_plan** plan);
 /* The same plan for the grouped layout: each order's elements contiguous

@@ -13,11 +13,11 @@
 // offsets at its position in the sorted list.  The single launch cuts each
 // order's part of the list into tiles of rounds x epb elements (epb =
 // elements per workgroup of that order's kernel body, chosen per order by
-// the tuner).  The persistent loop takes them by decreasing cost (workgroup
-// g takes tiles g, g + grid, ...: a static longest-first assignment, no
-// atomics, no counters to reset); one workgroup per tile takes them by the
-// memory position of their elements.  The per-order launches run each
-// order's part of the list with a kernel compiled for that order alone.
+// the tuner), ordered by the memory position of their elements.  The single
+// launch runs one workgroup per tile, or (persistent) a resident grid whose
+// workgroups take tiles from a queue in the plan.  The per-order launches
+// run each order's part of the list with a kernel compiled for that order
+// alone.
 // Both run the transposing BK3 body of bk13t_body.hpp.
 #include <hip/hip_runtime.h>
 
@@ -138,7 +138,9 @@ __device__ __forceinline__ void run_tile_p(int tid, const MixedTile& t, const in
 
 // LOOP = false: one workgroup per tile (a single launch over all orders; the
 // hardware dispatches the workgroups in tile order as CUs free up).
-// LOOP = true: persistent, workgroup g processes tiles g, g + gridDim.x, ...
+// LOOP = true: persistent, each workgroup takes the next tile from the queue
+// until none is left (two counters in device memory, zero between launches:
+// launches with one plan must not overlap).
 // In the persistent form the basis rows are read with vector loads (loads
 // after the loop's barriers and stores cannot be proven unclobbered, so they
 // do not become scalar loads), which costs registers and occupancy.
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
     int ntiles, const MixedTile* __restrict__ tiles, const int64_t* __restrict__ uoff,
     const int64_t* __restrict__ goff, const double* __restrict__ bases,
     const int64_t* __restrict__ boff, const double* __restrict__ G, const double* __restrict__ u,
-    double* __restrict__ v)
+    double* __restrict__ v, int* __restrict__ queue)
 {
     __shared__ double smem[kMixedSmem];
     auto one = [&](int k, int tid) {
@@ -162,13 +164,33 @@ __global__ __launch_bounds__(kMixedThreads) void bk3_mixed_kernel(
         }
     };
     if constexpr (LOOP) {
-        for (int k = blockIdx.x; k < ntiles; k += gridDim.x) {
+        // tiles taken from a queue: queue[0] counts the tiles taken, queue[1]
+        // the workgroups done; the last workgroup done resets both for the
+        // next launch on the stream
+        __shared__ int s_tile;
+        for (;;) {
+            if (threadIdx.x == 0) {
+                s_tile = atomicAdd(&queue[0], 1);
+            }
+            __syncthreads();
+            const int k = __builtin_amdgcn_readfirstlane(s_tile);
+            __syncthreads();
+            if (k >= ntiles) {
+                break;
+            }
             // The lane index, made to depend on the tile: the bodies' lane
             // arithmetic is then computed in the body that runs instead of
             // being hoisted out of the loop for all of them.
             int tid = threadIdx.x;
             asm("; tile %1" : "+v"(tid) : "s"(k));
             one(k, tid);
+        }
+        if (threadIdx.x == 0) {
+            __threadfence();
+            if (atomicAdd(&queue[1], 1) == int(gridDim.x) - 1) {
+                atomicExch(&queue[0], 0);
+                atomicExch(&queue[1], 0);
+            }
         }
     } else {
         if (int(blockIdx.x) < ntiles) {
@@ -232,6 +254,7 @@ struct bkx_mixed_plan {
     int64_t eoff_h[10] = {};            // per order: offset of B_p's even-odd form in the prepared array
     std::vector<int64_t> uoff_h, goff_h;
     bkx::MixedTile* tiles = nullptr;
+    int* queue = nullptr;       // the persistent launch's tile queue (two counters, zero between launches)
     int64_t* uoff = nullptr;    // by position in the order-sorted element list
     int64_t* goff = nullptr;
     int64_t* boff = nullptr;
@@ -268,10 +291,10 @@ int launch_tiles(const bkx::MixedTile* tiles, int ntiles, bool persistent, const
     const int64_t* off = EO ? pl->eoff : pl->boff;
     if (persistent) {
         hipLaunchKernelGGL((bkx::bk3_mixed_kernel<true, EO>), dim3(std::min(pl->grid, ntiles)), dim3(bkx::kMixedThreads),
-                           0, s, ntiles, tiles, pl->uoff, pl->goff, bases, off, G, in, out);
+                           0, s, ntiles, tiles, pl->uoff, pl->goff, bases, off, G, in, out, pl->queue);
     } else {
         hipLaunchKernelGGL((bkx::bk3_mixed_kernel<false, EO>), dim3(ntiles), dim3(bkx::kMixedThreads), 0, s, ntiles,
-                           tiles, pl->uoff, pl->goff, bases, off, G, in, out);
+                           tiles, pl->uoff, pl->goff, bases, off, G, in, out, pl->queue);
     }
     return bkx::launch_status();
 }
@@ -497,7 +520,6 @@ int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
 
     std::vector<int32_t> elems;
     std::vector<bkx::MixedTile> grouped;
-    std::vector<double> cost;
     for (int p = 1; p <= 8; ++p) {
         const int q = bkx::q_of(p);
         const int div = div_by_order ? div_by_order[p] : 1;
@@ -518,8 +540,6 @@ int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
         for (int t = 0; t < cnt; t += std::min(per_tile, cnt - t)) {
             const int c = std::min(per_tile, cnt - t);
             grouped.push_back(bkx::MixedTile{p, div, first + t, c});
-            const int n = bkx::n_of(p);
-            cost.push_back(double(c) * (2.0 * n * n * n + 6.0 * q * q * q));
         }
     }
     pl->elem_begin[9] = int(elems.size());
@@ -565,28 +585,25 @@ int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
     // persistent when asked for: rounds of more than one workgroup's
     // elements per tile, or a grid given without rounds.  Otherwise one
     // workgroup per tile, the hardware dispatching them in list order as CUs
-    // free up (on MI355X 59.7 GDoF/s for the 200,000-element Zipf batch in
-    // the caller's layout, against 50.5 for the persistent loop).
+    // free up (on MI355X 59.2 GDoF/s for the 200,000-element Zipf batch in
+    // the caller's layout, against 55.3 for the persistent loop).
     if (grid_given && rounds_by_order == nullptr) {
         pl->persistent = true;
     }
     pl->grid = std::max(1, std::min(grid, std::max(1, pl->ntiles)));
 
-    // Tile order.  Persistent: longest first, so that the static split
-    // (workgroup g takes tiles g, g + grid, ...) balances.  One workgroup
-    // per tile: by the memory position of the tile's first element, last
-    // first.  The workgroups in flight then work on neighbouring elements:
-    // in the caller's layout, which interleaves the orders, that ran at 59.7
-    // against 56.5 GDoF/s longest first.  In the grouped layout the orders
-    // lie in ascending order in memory, so this is longest first as well.
+    // Tile order: by the memory position of the tile's first element, last
+    // first, for both launch forms (the persistent one takes tiles from a
+    // queue, so any order balances).  The workgroups in flight then work on
+    // neighbouring elements: in the caller's layout, which interleaves the
+    // orders, the single launch ran at 59.2 against 56.7 GDoF/s longest
+    // first, and the persistent one at 55.3 against 50.7 with a static
+    // longest-first split.  In the grouped layout the orders lie in
+    // ascending order in memory, so this is longest first as well.
     std::vector<int> idx(grouped.size());
     std::iota(idx.begin(), idx.end(), 0);
-    if (pl->persistent) {
-        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    } else {
-        std::stable_sort(idx.begin(), idx.end(),
-                         [&](int a, int b) { return uoff_s[grouped[a].first] > uoff_s[grouped[b].first]; });
-    }
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int a, int b) { return uoff_s[grouped[a].first] > uoff_s[grouped[b].first]; });
     std::vector<bkx::MixedTile> ordered(grouped.size());
     for (size_t k = 0; k < idx.size(); ++k) {
         ordered[k] = grouped[idx[k]];
@@ -594,6 +611,8 @@ int plan_create(const int32_t* order, size_t nelmt, const int32_t* div_by_order,
 
     int st = BKX_SUCCESS;
     st = st ? st : upload(&pl->tiles, ordered.data(), ordered.size());
+    const int zero[2] = {0, 0};
+    st = st ? st : upload(&pl->queue, zero, 2);
     st = st ? st : upload(&pl->uoff, uoff_s.data(), nelmt);
     st = st ? st : upload(&pl->goff, goff_s.data(), nelmt);
     st = st ? st : upload(&pl->boff, boff, 10);
@@ -615,7 +634,7 @@ int bkx_mixed_plan_destroy(bkx_mixed_plan* plan)
     if (!plan) {
         return BKX_SUCCESS;
     }
-    for (void* ptr : {static_cast<void*>(plan->tiles),
+    for (void* ptr : {static_cast<void*>(plan->tiles), static_cast<void*>(plan->queue),
                       static_cast<void*>(plan->uoff),
                       static_cast<void*>(plan->goff), static_cast<void*>(plan->boff),
                       static_cast<void*>(plan->eoff)}) {

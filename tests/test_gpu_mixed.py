@@ -154,6 +154,34 @@ def test_graph_replays():
         graph.close()
 
 
+def test_persistent_queue_repeats():
+    """The persistent launch takes its tiles from a queue that each launch
+    leaves at zero: five eager applies in a row and four graph replays (with
+    eager work between them) equal the one-workgroup-per-tile launch, at a
+    small grid, at the resident grid, and with a grid clamped to the tile
+    count."""
+    order, plan, bases, G, u = _batch(4000, seed=5)
+    b, g, uu = _t(bases), _t(G), _t(u)
+    ref = torch.empty_like(uu)
+    plan.apply(b, g, uu, ref)
+    for kw in (dict(rounds_by_order=[2] * 9, grid=64), dict(rounds_by_order=[3] * 9), dict(grid=1 << 20)):
+        persist = mixed.MixedPlan(order, **kw)
+        v = torch.full_like(uu, float("nan"))
+        for rep in range(5):
+            v.fill_(float("nan"))
+            persist.apply(b, g, uu, v)
+            torch.cuda.synchronize()
+            assert torch.equal(v, ref), (kw, "eager", rep)
+        graph = persist.graph(b, g, uu, v)
+        for rep in range(4):
+            v.fill_(float("nan"))
+            graph.launch()
+            torch.cuda.synchronize()
+            assert torch.equal(v, ref), (kw, "graph", rep)
+            _ = (uu * 2.0).sum().item()      # eager work between replays
+        graph.close()
+
+
 def test_full_batch_sampled():
     """The 200,000-element Zipf batch: sampled elements of every order
     against the oracle; single launch == per-order launches."""
