@@ -217,7 +217,8 @@ def bk1_workload(p=3, E=4096):
     u = torch.rand(E * n ** 3, dtype=torch.float64, device=dev) * 2 - 1
     v = torch.empty_like(u)
     fn = lambda: ops.bk1(p, B, J, u, out=v)
-    return dict(fn=fn, out=v, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + E * q ** 3), p=p, E=E)
+    bound = ops.bind_bk1(p, B, J, u, out=v)
+    return dict(fn=fn, bound=bound, out=v, dofs=E * n ** 3, bytes=8 * (2 * E * n ** 3 + E * q ** 3), p=p, E=E)
 
 
 def geometry_lines(steps, warmup, p5=7, p3=7, E=65536):
@@ -731,6 +732,10 @@ def main():
             lines.update(bk3_even_odd(w, a.steps, a.warmup, ms))
         if name == "bk3_p7_f64":
             lines.update(bk3_reduced(w, a.steps, a.warmup, ms))
+        if "bound" in w:
+            # the same eager applies with the arguments checked once
+            bms, _ = time_steps(w["bound"], a.steps, a.warmup)
+            lines[name + "_bound"] = dict(rate(w, bms), host="arguments checked once (ops.bind_bk1)")
         if "out" in w:
             # 0.26 M DoF: the apply is shorter than its launch from the host
             gms, same = graph_ms(w, a.steps, a.warmup)

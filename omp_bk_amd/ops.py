@@ -65,6 +65,41 @@ def _on(t):
     return torch.cuda.device(idx)
 
 
+def _launch(entry, args, t):
+    """Enqueues a checked C-ABI call on the current stream of t's device."""
+    with _on(t):
+        _lib.call(entry, *args, _stream(t))
+
+
+class Bound:
+    """One operator apply whose arguments were checked once (bind_bk1,
+    bind_bk3, bind_bk5).  Each call enqueues the kernel on the current stream
+    of the arrays' device with the same arrays, which the object keeps
+    alive; their contents may change between calls, their storage must not.
+    For loops over one small operator, where the plain wrappers' per-call
+    checks take longer on the host than the apply takes on the device.
+    Returns the output array."""
+
+    def __init__(self, entry, args, arrays, out):
+        self._entry = entry
+        self._fn = getattr(_lib.lib(), entry)
+        self._args = args
+        self._arrays = arrays      # the pointers in _args are theirs
+        self._dev = out.get_device()
+        self.out = out
+
+    def __call__(self):
+        dev = self._dev
+        if dev == torch.cuda.current_device():
+            status = self._fn(*self._args, _stream(self.out))
+        else:
+            with torch.cuda.device(dev):
+                status = self._fn(*self._args, _stream(self.out))
+        if status != 0:
+            raise _lib.BkxError(self._entry, status, _lib.lib().bkx_status_string(status).decode())
+        return self.out
+
+
 def ctypes_stream(stream):
     return stream.cuda_stream
 
@@ -104,6 +139,14 @@ def bk1(p, basis, JxW, u, out=None, form="f64"):
                openmp/BK1_ff.cpp:73-723), v FP64;
          "f32c": u, JxW in FP32, float-float compensated arithmetic, v FP64;
          "f32w": u, JxW in FP32, FP64 arithmetic, v FP64."""
+    entry, args, out = _bk1_call(p, basis, JxW, u, out, form)
+    _launch(entry, args, u)
+    return out
+
+
+def _bk1_call(p, basis, JxW, u, out, form):
+    """bk1's checks: its C-ABI entry point, the arguments before the stream,
+    and out (allocated when None)."""
     _order_ok(1, p)
     entry, sdt = _bk13_form(1, form)
     nm, nq = p + 1, p + 2
@@ -117,9 +160,7 @@ def bk1(p, basis, JxW, u, out=None, form="f64"):
         out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nm ** 3)
     _apart(out, u)
-    with _on(u):
-        _lib.call(entry, p, ne, basis.data_ptr(), JxW.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
-    return out
+    return entry, (p, ne, basis.data_ptr(), JxW.data_ptr(), u.data_ptr(), out.data_ptr()), out
 
 
 def bk3(p, basis, dbasis, G, u, out=None, form="f64"):
@@ -130,6 +171,13 @@ def bk3(p, basis, dbasis, G, u, out=None, form="f64"):
                hi + lo when loaded), v FP64;
          "f32c": u, G in FP32, float-float compensated arithmetic, v FP64;
          "f32w": u, G in FP32, FP64 arithmetic, v FP64."""
+    entry, args, out = _bk3_call(p, basis, dbasis, G, u, out, form)
+    _launch(entry, args, u)
+    return out
+
+
+def _bk3_call(p, basis, dbasis, G, u, out, form):
+    """bk3's checks, as _bk1_call."""
     _order_ok(3, p)
     entry, sdt = _bk13_form(3, form)
     nm, nq = p + 1, p + 2
@@ -144,10 +192,7 @@ def bk3(p, basis, dbasis, G, u, out=None, form="f64"):
         out = torch.empty(u.shape, dtype=torch.float64, device=u.device)
     _check("out", out, ne * nm ** 3)
     _apart(out, u)
-    with _on(u):
-        _lib.call(entry, p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(),
-                  _stream(u))
-    return out
+    return entry, (p, ne, basis.data_ptr(), dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr()), out
 
 
 def bk3_basis(p, basis, dbasis, device=None):
@@ -218,6 +263,13 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
                   FP32 (p = 7 only), v FP32;
          "oz": u, G in FP32, contractions from exact 8-bit digit products on
                BF16 MFMA with exact FP32 level sums (p = 5, 6, 7), v FP64."""
+    entry, args, out = _bk5_call(p, dbasis, G, u, out, form)
+    _launch(entry, args, u)
+    return out
+
+
+def _bk5_call(p, dbasis, G, u, out, form):
+    """bk5's checks, as _bk1_call."""
     _order_ok(5, p)
     if form not in _BK5_FORMS:
         raise ValueError("unknown BK5 form %r (one of %s)" % (form, ", ".join(_BK5_FORMS)))
@@ -234,9 +286,25 @@ def bk5(p, dbasis, G, u, out=None, form="f64"):
         out = torch.empty(u.shape, dtype=odt, device=u.device)
     _check("out", out, ne * nq ** 3, odt)
     _apart(out, u)
-    with _on(u):
-        _lib.call(entry, p, ne, dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr(), _stream(u))
-    return out
+    return entry, (p, ne, dbasis.data_ptr(), G.data_ptr(), u.data_ptr(), out.data_ptr()), out
+
+
+def bind_bk1(p, basis, JxW, u, out=None, form="f64"):
+    """bk1 with its arguments checked once: a Bound whose calls apply it."""
+    entry, args, out = _bk1_call(p, basis, JxW, u, out, form)
+    return Bound(entry, args, (basis, JxW, u, out), out)
+
+
+def bind_bk3(p, basis, dbasis, G, u, out=None, form="f64"):
+    """bk3 with its arguments checked once: a Bound whose calls apply it."""
+    entry, args, out = _bk3_call(p, basis, dbasis, G, u, out, form)
+    return Bound(entry, args, (basis, dbasis, G, u, out), out)
+
+
+def bind_bk5(p, dbasis, G, u, out=None, form="f64"):
+    """bk5 with its arguments checked once: a Bound whose calls apply it."""
+    entry, args, out = _bk5_call(p, dbasis, G, u, out, form)
+    return Bound(entry, args, (dbasis, G, u, out), out)
 
 
 _KAPPA_FORMS = {"f64": torch.float64, "f32w": torch.float32, "f32c": torch.float32}

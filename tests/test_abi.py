@@ -156,3 +156,18 @@ def test_order_shape_checks_without_device():
     d, g = ctypes.c_int(), ctypes.c_int()
     assert M.bkx_mixed_plan_set_order_shape(None, 3, 1, 3) == 2
     assert M.bkx_mixed_plan_get_order_shape(None, 3, ctypes.byref(d), ctypes.byref(g)) == 2
+
+
+def test_bind_checks_arguments_without_device():
+    """A bound apply runs the plain wrapper's checks when it is made: host
+    tensors and short arrays are refused before any device call."""
+    import torch
+    from omp_bk_amd import ops
+    p = 3
+    B = torch.zeros(4 * 5, dtype=torch.float64)
+    J = torch.zeros(2 * 125, dtype=torch.float64)
+    u = torch.zeros(2 * 64, dtype=torch.float64)
+    with pytest.raises(ValueError, match="HIP device"):
+        ops.bind_bk1(p, B, J, u, out=torch.zeros_like(u))
+    with pytest.raises(ValueError, match="unsupported polynomial order"):
+        ops.bind_bk5(8, B, J, u)
