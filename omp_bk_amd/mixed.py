@@ -170,7 +170,9 @@ class MixedPlan:
     grouped, the batch's arrays are in the grouped layout
     (bkx_mixed_plan_create_grouped): each order's elements contiguous, each
     order's range on a 256-byte boundary; uoff and goff give every element's
-    place either way."""
+    place either way.  With rounds_by_order or a grid the single launch is
+    persistent: its workgroups take tiles from a queue held by the plan, so
+    launches of one such plan (direct or replayed) must not overlap."""
 
     def __init__(self, order, div_by_order=None, rounds_by_order=None, grid=0, grouped=False):
         L = _bind()
